@@ -1,0 +1,198 @@
+/*
+ * qsim.h — C ABI of libqsim_hip.so, the MI355X (gfx950) circuit-simulation
+ * library behind quict_amd.
+ *
+ * The reference's drop-in boundary for this path is the function table that
+ * QuICT binds through LinAlgLoader (QuICT/ops/utils/algorithm_loader.py:27-60):
+ * the per-gate kernels exported by QuICT/ops/gate_kernel/gate_function.py:6-28
+ * plus QuICT/ops/linalg/gpu_calculator.py (dot, matrix_dot_vector, ...), each
+ * called in place on a device vector with reversed qubit indices (n-1-q).  The
+ * entry points below replace that table; each one names the reference
+ * function(s) it stands in for.  A maintainer binds them from Python with
+ * ctypes (see quict_amd/_native.py and INTEGRATION.md).
+ *
+ * Conventions
+ *  - Every function returns an int status: QS_OK (0) or a negative QS_ERR_*.
+ *    qs_last_error() returns a message for the most recent failure on the
+ *    calling thread.
+ *  - Amplitude/density buffers are caller-owned device memory (for example a
+ *    torch tensor's data_ptr()).  The library never frees caller memory.
+ *  - "bits" are amplitude-index bit positions (0 = least significant).  QuICT
+ *    qubit q of an n-qubit register is bit n-1-q.
+ *  - stream is a hipStream_t (NULL = the null stream).  Calls that return host
+ *    values synchronise that stream; all others are asynchronous.
+ *  - A context is bound to one device and is not thread-safe.
+ */
+#ifndef QSIM_H
+#define QSIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QS_OK 0
+#define QS_ERR_ARG (-1)         /* invalid argument: bad bit index, kind, size, null pointer */
+#define QS_ERR_DEVICE (-2)      /* HIP runtime failure: launch error, out of memory */
+#define QS_ERR_UNSUPPORTED (-3) /* valid request this build does not implement */
+
+#define QS_C64 0  /* complex<float>  amplitudes, interleaved (re, im) */
+#define QS_C128 1 /* complex<double> amplitudes, interleaved (re, im) */
+
+#define QS_MAX_GATE_BITS 16
+#define QS_MAX_DENSE_TARGETS 10
+
+/* Gate kinds.  Matrix payload m[] holds complex numbers as (re, im) doubles. */
+enum qs_gate_kind {
+  QS_GATE_DENSE = 0, /* 2^t x 2^t dense on the targets (row-major), applied where all
+                        controls are 1.  t <= 2: m[]; t >= 3: big (host pointer). */
+  QS_GATE_DIAG = 1,  /* diagonal, 2^t entries in m[] (t <= 4), under controls */
+  QS_GATE_PHASE = 2, /* multiply the amplitude whose controls and targets are all 1 by m[0] */
+  QS_GATE_XPERM = 3, /* 1 target: a0' = m[0]*a1, a1' = m[1]*a0, under controls (X, Y, CX, CY, CCX) */
+  QS_GATE_SWAP = 4   /* 2 targets: a01' = m[0]*a10, a10' = m[1]*a01, under controls (Swap, iSwap, CSwap) */
+};
+
+typedef struct qs_gate {
+  int32_t kind;
+  int32_t ncontrols;
+  int32_t ntargets;
+  int32_t bits[QS_MAX_GATE_BITS]; /* controls first, then targets; target 0 = MSB of the matrix index */
+  double m[32];                   /* kind-specific payload (up to 16 complex) */
+  const double* big;              /* DENSE, ntargets >= 3: host 2^t x 2^t complex, row-major */
+} qs_gate;
+
+typedef struct qs_ctx qs_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Create a context on HIP device `device`.  Returns NULL on failure. */
+qs_ctx* qs_ctx_create(int device);
+void qs_ctx_destroy(qs_ctx* ctx);
+const char* qs_last_error(void);
+/* Bytes of device scratch the context currently holds (measurement pyramids,
+   fusion programs, reduction partials). */
+uint64_t qs_ctx_scratch_bytes(const qs_ctx* ctx);
+
+/* Kernel timing for measurement: when enabled, every tile-pass / per-gate /
+   gradient / batch launch is bracketed by HIP events on its own stream.
+   qs_profile_read returns, per class [fused tile pass, per-gate pass,
+   adjoint, qnn batch], the summed kernel milliseconds and launch counts. */
+int qs_profile_enable(qs_ctx* ctx, int on);
+int qs_profile_read(qs_ctx* ctx, double* ms /*[4]*/, int64_t* launches /*[4]*/, int reset);
+
+/* ---- state vector ------------------------------------------------------- */
+/* |index> : replaces StateVectorSimulator.initial_state_vector
+   (QuICT/simulation/state_vector/statevector_simulator.py:122-129). */
+int qs_sv_init(qs_ctx* ctx, void* vec, int n, int dtype, uint64_t index, void* stream);
+
+/* One gate, one full-state pass (no fusion).  Replaces the per-gate table of
+   QuICT/ops/gate_kernel/gate_function.py:1034-1590 (diagonal_targ, normal_targ,
+   control_targ, swap_targ, reverse_targ, *_targs, *_ctargs, reverse_more,
+   diagonal_more, swap_tmore) and, for >= 3 targets,
+   QuICT/ops/linalg/gpu_calculator.py:430-478 (matrix_dot_vector). */
+int qs_sv_apply_gate(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gate, void* stream);
+
+/* A gate sequence.  fuse=0: one pass per gate (same kernels as qs_sv_apply_gate).
+   fuse=1: the fusion planner groups gates into LDS/register-resident tiles and
+   runs each group in one read+write pass.  Replaces the per-gate loop of
+   StateVectorSimulator.run (statevector_simulator.py:155-164, 202-315).
+   *passes (may be NULL) receives the number of full-state passes executed. */
+int qs_sv_apply_gates(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
+                      int fuse, int* passes, void* stream);
+
+/* Fusion tile shape: log2 amplitudes per tile (0 = built-in default) and the
+   number of low bits always kept contiguous.  Set by the autotuner. */
+int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
+
+/* P(bit == 0) = sum |a|^2 over the half with that bit clear, reduced in double.
+   Replaces measured_prob_calculate (gate_function.py:1751-1776). */
+int qs_sv_prob_zero(qs_ctx* ctx, const void* vec, int n, int dtype, int bit, double* p0, void* stream);
+
+/* Keep the `outcome` half scaled by `scale`, zero the other half.  Replaces the
+   MeasureGate0/1 kernels of apply_measuregate (gate_function.py:1779-1815). */
+int qs_sv_collapse(qs_ctx* ctx, void* vec, int n, int dtype, int bit, int outcome, double scale, void* stream);
+
+/* Reset `bit` to 0 given p0 = P(bit == 0) (reset_gate_apply semantics,
+   QuICT/ops/linalg/cpu_calculator.py:366-384). */
+int qs_sv_reset(qs_ctx* ctx, void* vec, int n, int dtype, int bit, double p0, void* stream);
+
+/* Sample `shots` full-register outcomes.  uniforms: host (shots x n) doubles,
+   row s holds the draws for qubits 0..n-1 (bit n-1 first).  Outcome rule per
+   qubit: 1 iff u > P(0 | outcomes so far), the rule of
+   StateVectorSimulator.sample (statevector_simulator.py:595-626).  out: host
+   uint64 amplitude indices.  Builds a pair-sum pyramid of |a|^2 in double. */
+int qs_sv_sample(qs_ctx* ctx, const void* vec, int n, int dtype, const double* uniforms, int shots,
+                 uint64_t* out, void* stream);
+
+/* Same sampling rule over a caller device buffer of 2^n probabilities (used
+   for the diagonal of a density matrix, DensityMatrixSimulator.sample,
+   QuICT/simulation/density_matrix/density_matrix_simulator.py:210-240). */
+int qs_sample_probs(qs_ctx* ctx, const double* probs_dev, int n, const double* uniforms, int shots, uint64_t* out,
+                    void* stream);
+
+/* |a_i|^2 into a caller device buffer of 2^n doubles. */
+int qs_sv_probabilities(qs_ctx* ctx, const void* vec, int n, int dtype, double* probs_dev, void* stream);
+
+/* sum |a|^2 (host result). */
+int qs_sv_norm2(qs_ctx* ctx, const void* vec, int n, int dtype, double* out, void* stream);
+
+/* Qubit permutation, out of place: out[j] = vec[src(j)] with
+   bit perm[b] of src taken from bit b of j.  Replaces VectorPermutation
+   (QuICT/ops/linalg/gpu_calculator.py:170-250). */
+int qs_sv_permute(qs_ctx* ctx, const void* vec, void* out, int n, int dtype, const int32_t* perm, void* stream);
+
+/* ---- density matrix (vec(rho) as a 2n-bit state: row bits n..2n-1) ------- */
+/* diag(rho) into a caller device buffer of 2^n doubles (real parts). */
+int qs_dm_diagonal(qs_ctx* ctx, const void* rho, int n, int dtype, double* diag_dev, void* stream);
+/* trace (host). */
+int qs_dm_trace(qs_ctx* ctx, const void* rho, int n, int dtype, double* re, double* im, void* stream);
+/* P(qubit bit == 0) from the diagonal (host).  Replaces the projector trace of
+   DensityMatrixSimulator.apply_measure (density_matrix_simulator.py:182-208). */
+int qs_dm_prob_zero(qs_ctx* ctx, const void* rho, int n, int dtype, int bit, double* p0, void* stream);
+/* rho <- P rho P / p: zero rows/cols whose `bit` != outcome, scale the rest. */
+int qs_dm_collapse(qs_ctx* ctx, void* rho, int n, int dtype, int bit, int outcome, double scale, void* stream);
+
+/* ---- expectation values and gradients (no reference counterpart) --------- */
+/* Observable sum_k coef_k * P_k with P_k a Pauli string given as two bit masks
+   (x-mask: bits carrying X or Y, z-mask: bits carrying Z or Y). */
+typedef struct qs_pauli_term {
+  double coef;
+  uint64_t xmask;
+  uint64_t zmask;
+} qs_pauli_term;
+
+/* out <- H vec (caller device buffer, same size/dtype as vec). */
+int qs_sv_apply_pauli_sum(qs_ctx* ctx, const void* vec, void* out, int n, int dtype, const qs_pauli_term* terms,
+                          int nterms, void* stream);
+/* <a|b> (host, complex). */
+int qs_sv_inner(qs_ctx* ctx, const void* a, const void* b, int n, int dtype, double* re, double* im, void* stream);
+
+/* Adjoint-method gradient of E = <psi|H|psi> for psi = G_{L-1}...G_0 |psi_in>.
+   On entry psi holds the forward state, lam holds H psi.  For gate j with
+   param_slot[j] >= 0, gen[j] is its generator as a (re, im) 2^t x 2^t matrix D_j
+   with dG_j/dtheta = D_j (stored in gen_mats at offset gen_off[j]).  The
+   routine walks j = L-1..0 applying G_j^dagger to psi and lam and
+   accumulating grads[param_slot[j]] += 2 Re <lam| D_j |psi_before_j>.
+   No per-layer states are stored: psi is recomputed by inverse gates.
+   psi and lam are overwritten. grads: host, nslots doubles (accumulated). */
+int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_gate* gates, int ngates,
+                        const qs_gate* dgates, const int32_t* param_slot, double* grads, int nslots, void* stream);
+
+/* ---- batched variational circuits (no reference counterpart) ------------- */
+/* Hardware-efficient-ansatz QNN batch, one persistent launch for forward and
+   adjoint backward of every circuit.  Circuit c has width widths[c] (<= 18),
+   angle-encodes x[c*nfeat + j] with Ry on qubit j mod width, then `layers`
+   layers of (Ry(theta), Rz(theta) on every qubit; CX(i, i+1) ladder).  theta is
+   shared: theta[(l*2 + g)*max_width + q].  Loss per circuit:
+   (<Z_0> - target[c])^2.  Outputs (host): expval[c], loss_grad_theta (summed
+   over circuits, scaled by 1/ncirc as for a mean loss), grad_x (per circuit,
+   nfeat).  dtype selects c64/c128 arithmetic. */
+int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int nfeat, const double* x,
+                 const double* target, int layers, int max_width, const double* theta, double* expval,
+                 double* loss, double* grad_theta, double* grad_x, int tile_log2, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSIM_H */

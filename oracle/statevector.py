@@ -1,0 +1,241 @@
+"""State-vector oracle (CPU, numpy).  Test infrastructure only.
+
+Restates QuICT/ops/linalg/cpu_calculator.py:139-384 and the CPU dispatch of
+QuICT/simulation/state_vector/statevector_simulator.py:168-220,595-626:
+
+  * qubit q lives at amplitude-index bit n-1-q;
+  * per gate, ``matrix_type`` picks the update: diagonal / control / diag_diag
+    -> elementwise multiply of the touched amplitudes (diagonal_matrix,
+    :189-235); swap -> exchange (swap_matrix, :238-280); reverse -> anti-diagonal
+    (reverse_matrix, :283-326); everything else -> gather / small mat-vec /
+    scatter per group of 2^k amplitudes (matrix_dot_vector, :139-186);
+  * measurement: P0 = sum |a|^2 over bit=0, u = random.random(), outcome =
+    u > P0, survivors scaled by 1/sqrt(P) (measure_gate_apply, :329-363);
+  * reset (reset_gate_apply, :366-384).
+
+Deliberate differences from the reference, each a reference defect:
+  * gate-matrix argument order is ``cargs + targs`` with argument 0 the MSB for
+    every gate (the convention of circuit.matrix() and of the reference's GPU
+    path for >= 3 targets).  The reference CPU path feeds ``targs + cargs``
+    with argument 0 as LSB, which only changes results for asymmetric
+    multi-target dense gates (Rzx, 2-target Unitary) — none appear in the
+    golden circuit or the benchmark workloads;
+  * iSwap-family phases are applied (the reference CPU swap ignores them);
+  * a diagonal gate as wide as the register is applied (the reference CPU
+    path rebinds a local and drops it).
+Sampling uses CPython's ``random.random()`` — the generator the reference's
+GPU path and density-matrix path draw from (gate_function.py:1795,
+density_matrix_simulator.py:191).
+"""
+import random
+from typing import List
+
+import numpy as np
+
+from quict_amd.core import Circuit, BasicGate, GateType, MatrixType, Trigger
+
+
+def _group_indices(n: int, bits: List[int]) -> np.ndarray:
+    """Base index (all ``bits`` zero) of every group, ascending.
+
+    Same insertion as cpu_calculator.py:179-183: walk the sorted bit list and
+    open a zero at each position.
+    """
+    k = len(bits)
+    g = np.arange(1 << (n - k), dtype=np.int64)
+    for b in sorted(bits):
+        g = ((g >> b) << (b + 1)) | (g & ((1 << b) - 1))
+    return g
+
+
+def _offsets(bits_msb_first: List[int]) -> np.ndarray:
+    """Offset of gate-matrix index m (arg 0 = MSB) inside a group."""
+    k = len(bits_msb_first)
+    off = np.zeros(1 << k, dtype=np.int64)
+    for m in range(1 << k):
+        for t, b in enumerate(bits_msb_first):
+            if (m >> (k - 1 - t)) & 1:
+                off[m] |= 1 << b
+    return off
+
+
+def apply_matrix(vec: np.ndarray, n: int, mat: np.ndarray, bits_msb_first: List[int]):
+    """Dense 2^k x 2^k ``mat`` on amplitude bits (matrix_dot_vector, :139-186)."""
+    base = _group_indices(n, bits_msb_first)
+    idx = base[:, None] + _offsets(bits_msb_first)[None, :]
+    vec[idx] = vec[idx] @ mat.T.astype(vec.dtype)
+
+
+def apply_diagonal(vec: np.ndarray, n: int, diag: np.ndarray, bits_msb_first: List[int]):
+    base = _group_indices(n, bits_msb_first)
+    idx = base[:, None] + _offsets(bits_msb_first)[None, :]
+    vec[idx] = vec[idx] * diag.astype(vec.dtype)[None, :]
+
+
+def apply_gate(vec: np.ndarray, n: int, gate: BasicGate):
+    """One gate, dispatched on ``matrix_type`` as in statevector_simulator.py:168-193."""
+    gt = gate.type
+    if gt in (GateType.barrier, GateType.id):
+        return
+    if gt == GateType.measure:
+        raise ValueError("use measure()")
+    if gt == GateType.reset:
+        reset(vec, n, n - 1 - gate.targ)
+        return
+    if gt == GateType.perm:
+        apply_perm(vec, n, gate)
+        return
+    args = gate.cargs + gate.targs
+    bits = [n - 1 - a for a in args]
+    mat = np.asarray(gate.matrix)
+    mt = gate.matrix_type
+    if mt in (MatrixType.diagonal, MatrixType.diag_diag, MatrixType.control):
+        apply_diagonal(vec, n, np.diag(mat).copy(), bits)
+    elif mt == MatrixType.swap and gate.targets == 1 and gate.controls == 0:
+        # X: exchange the pair (swap_matrix, :238-280)
+        base = _group_indices(n, bits)
+        i1 = base + (1 << bits[0])
+        vec[base], vec[i1] = vec[i1].copy(), vec[base].copy()
+    elif mt == MatrixType.swap:
+        # controlled exchange of |..01> and |..10> with the matrix's phases
+        d = 1 << len(args)
+        base = _group_indices(n, bits)
+        off = _offsets(bits)
+        ia, ib = base + off[d - 3], base + off[d - 2]
+        va, vb = vec[ia].copy(), vec[ib].copy()
+        vec[ia] = mat[d - 3, d - 2] * vb
+        vec[ib] = mat[d - 2, d - 3] * va
+    elif mt == MatrixType.reverse:
+        # anti-diagonal on the target inside the all-controls-one block (reverse_matrix, :283-326)
+        d = 1 << len(args)
+        base = _group_indices(n, bits)
+        off = _offsets(bits)
+        ia, ib = base + off[d - 2], base + off[d - 1]
+        va, vb = vec[ia].copy(), vec[ib].copy()
+        vec[ia] = mat[d - 2, d - 1] * vb
+        vec[ib] = mat[d - 1, d - 2] * va
+    else:
+        apply_matrix(vec, n, mat, bits)
+
+
+def apply_perm(vec: np.ndarray, n: int, gate: BasicGate):
+    """Perm gate dispatch of statevector_simulator.py:297-312: mapping[args[i]] = args[pargs[i]]."""
+    args = gate.cargs + gate.targs
+    if not args:
+        args = list(range(n))
+    mapping = list(range(n))
+    for i, p in enumerate(gate.pargs):
+        mapping[args[i]] = args[p]
+    # VectorPermutation (QuICT/ops/linalg/gpu_calculator.py:170-207):
+    # source qubit i takes the output's qubit mapping[i].
+    idx = np.arange(1 << n, dtype=np.int64)
+    src = np.zeros_like(idx)
+    for i in range(n):
+        src |= ((idx >> (n - 1 - mapping[i])) & 1) << (n - 1 - i)
+    vec[:] = vec[src]
+
+
+def prob_zero(vec: np.ndarray, n: int, bit: int) -> float:
+    """P(amplitude-index ``bit`` == 0), summed in the vector's precision."""
+    a = vec.reshape(-1, 2, 1 << bit)[:, 0, :]
+    return np.sum(np.abs(a) ** 2)
+
+
+def collapse(vec: np.ndarray, n: int, bit: int, outcome: int, prob: float):
+    v = vec.reshape(-1, 2, 1 << bit)
+    rdt = np.float32 if vec.dtype == np.complex64 else np.float64
+    if outcome:
+        alpha = rdt(1 / np.sqrt(1 - prob))
+        v[:, 0, :] = 0
+        v[:, 1, :] *= alpha
+    else:
+        alpha = rdt(1 / np.sqrt(prob))
+        v[:, 0, :] *= alpha
+        v[:, 1, :] = 0
+
+
+def measure(vec: np.ndarray, n: int, bit: int, rng=random) -> int:
+    """measure_gate_apply (cpu_calculator.py:329-363) with CPython's ``random``."""
+    p0 = prob_zero(vec, n, bit)
+    outcome = int(rng.random() > p0)
+    collapse(vec, n, bit, outcome, p0)
+    return outcome
+
+
+def reset(vec: np.ndarray, n: int, bit: int):
+    """reset_gate_apply (cpu_calculator.py:366-384)."""
+    v = vec.reshape(-1, 2, 1 << bit)
+    p0 = np.sum(np.abs(v[:, 0, :]) ** 2)
+    alpha = np.sqrt(p0)
+    if alpha < 1e-6:
+        v[:, 0, :] = v[:, 1, :]
+        v[:, 1, :] = 0
+    else:
+        v[:, 0, :] /= alpha
+        v[:, 1, :] = 0
+
+
+def initial_state(n: int, dtype=np.complex128) -> np.ndarray:
+    v = np.zeros(1 << n, dtype=dtype)
+    v[0] = 1
+    return v
+
+
+def run(circuit: Circuit, state_vector: np.ndarray = None, dtype=np.complex128, rng=random) -> np.ndarray:
+    """Whole circuit (statevector_simulator.py:131-166): gates, measure, reset, triggers."""
+    n = circuit.width()
+    vec = initial_state(n, dtype) if state_vector is None else np.array(state_vector, dtype=dtype)
+    pipeline = list(circuit.gates)
+    i = 0
+    while i < len(pipeline):
+        g = pipeline[i]
+        i += 1
+        if isinstance(g, Trigger):
+            for q in g.targs:
+                circuit.qubits[q].measured = measure(vec, n, n - 1 - q, rng)
+            cg = g.mapping(int(circuit.qubits[g.targs]))
+            if cg is not None:
+                gates = cg.gates if hasattr(cg, "gates") else [cg]
+                pipeline = pipeline[:i] + list(gates) + pipeline[i:]
+            continue
+        if g.type == GateType.measure:
+            circuit.qubits[g.targ].measured = measure(vec, n, n - 1 - g.targ, rng)
+            continue
+        apply_gate(vec, n, g)
+    return vec
+
+
+def sample(vec: np.ndarray, n: int, shots: int, rng=random) -> List[int]:
+    """statevector_simulator.py:595-626: per shot measure qubits 0..n-1 in order,
+    count the big-endian readout, restore the state."""
+    counts = [0] * (1 << n)
+    for _ in range(shots):
+        v = vec.copy()
+        value = 0
+        for q in range(n):
+            value = (value << 1) | measure(v, n, n - 1 - q, rng)
+        counts[value] += 1
+    return counts
+
+
+def sample_outcomes(vec: np.ndarray, n: int, uniforms: np.ndarray) -> np.ndarray:
+    """Same decision rule as ``sample`` but driven by a given (shots, n) array of
+    uniforms and computed by conditional masses (used to cross-check the GPU
+    tree-descent sampler without re-running the collapse per shot)."""
+    p = np.abs(vec.astype(np.complex128)) ** 2
+    # pyramid of pair sums: levels[l] has 2^(n-l) entries
+    levels = [p]
+    for _ in range(n):
+        levels.append(levels[-1].reshape(-1, 2).sum(axis=1))
+    out = np.zeros(len(uniforms), dtype=np.int64)
+    for s, u in enumerate(uniforms):
+        node = 0
+        for q in range(n):
+            lvl = levels[n - 1 - q]
+            left, right = lvl[2 * node], lvl[2 * node + 1]
+            total = left + right
+            p0 = left / total if total > 0 else 1.0
+            bit = 1 if u[q] > p0 else 0
+            node = 2 * node + bit
+        out[s] = node
+    return out
