@@ -81,8 +81,12 @@ template <typename R> __global__ void k_init(cx<R>* __restrict__ v, uint64_t N, 
 }
 
 // out[j] = in[src(j)], bit perm[b] of src = bit b of j.
+struct PermBits {
+  int b[64];
+};
+
 template <typename R>
-__global__ void k_permute(const cx<R>* __restrict__ in, cx<R>* __restrict__ out, uint64_t N, int n, SortedBits pm) {
+__global__ void k_permute(const cx<R>* __restrict__ in, cx<R>* __restrict__ out, uint64_t N, int n, PermBits pm) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   uint64_t src = 0;
@@ -365,16 +369,13 @@ int qs_sv_permute(qs_ctx* ctx, const void* vec, void* out, int n, int dtype, con
   int rc = check_state_args(vec, n, dtype);
   if (rc) return rc;
   QS_REQUIRE(ctx && out && perm && out != vec, "bad arguments (out must differ from vec)");
-  QS_REQUIRE(n <= QS_MAX_GATE_BITS * 4, "register too wide");
-  SortedBits pm;
-  pm.n = n;
+  PermBits pm;
   uint64_t seen = 0;
   for (int b = 0; b < n; ++b) {
     QS_REQUIRE(perm[b] >= 0 && perm[b] < n && !(seen & (1ull << perm[b])), "perm must be a permutation");
     seen |= 1ull << perm[b];
-    if (b < QS_MAX_GATE_BITS) pm.b[b] = perm[b];
+    pm.b[b] = perm[b];
   }
-  QS_REQUIRE(n <= QS_MAX_GATE_BITS, "permutation supports at most 16 bits");
   const uint64_t N = 1ull << n;
   const unsigned blocks = (unsigned)((N + 255) / 256);
   if (dtype == QS_C128)
