@@ -179,17 +179,30 @@ int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, con
                         const qs_gate* dgates, const int32_t* param_slot, double* grads, int nslots, void* stream);
 
 /* ---- batched variational circuits (no reference counterpart) ------------- */
+/* Launch shape of the batched QNN kernel (picked by the autotuner):
+   widths <= lds_max_width keep psi and lam in LDS (2 x 2^w amplitudes),
+   wider circuits use a global slot per workgroup; block_threads threads per
+   workgroup; blocks_per_cu persistent workgroups per CU; use_graph != 0
+   captures the launch in a hipGraph on first use and replays it for every
+   later batch of the same shape. */
+typedef struct qs_qnn_config {
+  int32_t lds_max_width;
+  int32_t block_threads;
+  int32_t blocks_per_cu;
+  int32_t use_graph;
+} qs_qnn_config;
+
 /* Hardware-efficient-ansatz QNN batch, one persistent launch for forward and
-   adjoint backward of every circuit.  Circuit c has width widths[c] (<= 18),
+   adjoint backward of every circuit.  Circuit c has width widths[c] (<= 20),
    angle-encodes x[c*nfeat + j] with Ry on qubit j mod width, then `layers`
    layers of (Ry(theta), Rz(theta) on every qubit; CX(i, i+1) ladder).  theta is
-   shared: theta[(l*2 + g)*max_width + q].  Loss per circuit:
-   (<Z_0> - target[c])^2.  Outputs (host): expval[c], loss_grad_theta (summed
-   over circuits, scaled by 1/ncirc as for a mean loss), grad_x (per circuit,
-   nfeat).  dtype selects c64/c128 arithmetic. */
+   shared: theta[(l*2 + g)*max_width + q].  Loss: mean over circuits of
+   (<Z_0> - target[c])^2.  Outputs (host): expval[c], loss, grad_theta
+   (d loss / d theta), grad_x (per circuit, nfeat; may be NULL).  dtype selects
+   complex64 / complex128 amplitudes (reductions in double).  cfg may be NULL. */
 int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int nfeat, const double* x,
                  const double* target, int layers, int max_width, const double* theta, double* expval,
-                 double* loss, double* grad_theta, double* grad_x, int tile_log2, void* stream);
+                 double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,61 @@
+"""Gradient oracle (CPU, numpy).  Test infrastructure only.
+
+Pinned by closed forms (Ry(theta)|0>: <Z> = cos(theta), d/dtheta = -sin(theta))
+and by the parameter-shift rule, which is exact for gates exp(-i theta P / 2)
+with P a Pauli string (QuICT/core/gate/gate.py Rx/Ry/Rz/CRz/Rxx/Ryy/Rzz/Rzx).
+"""
+import numpy as np
+
+from . import statevector as osv
+
+
+def pauli_apply(vec: np.ndarray, n: int, terms) -> np.ndarray:
+    """H vec for H = sum_k c_k P_k (P_k as 'IXYZ' strings, qubit 0 first)."""
+    out = np.zeros_like(vec, dtype=np.complex128)
+    idx = np.arange(1 << n, dtype=np.int64)
+    for c, s in terms:
+        x = z = 0
+        ny = 0
+        for q, ch in enumerate(s):
+            bit = 1 << (n - 1 - q)
+            if ch in "XY":
+                x |= bit
+            if ch in "ZY":
+                z |= bit
+            if ch == "Y":
+                ny += 1
+        src = idx ^ x
+        sign = np.array([(-1) ** bin(int(i) & z).count("1") for i in src]) if n <= 12 else \
+            (-1.0) ** np.array([bin(int(i) & z).count("1") for i in src])
+        out += c * (1j ** ny) * sign * vec[src]
+    return out
+
+
+def expectation(pc, theta, terms) -> float:
+    c = pc.bind(np.asarray(theta, dtype=np.float64))
+    v = osv.run(c)
+    return float(np.real(np.vdot(v, pauli_apply(v, pc.n, terms))))
+
+
+def parameter_shift(pc, theta, terms, params=None) -> np.ndarray:
+    theta = np.asarray(theta, dtype=np.float64)
+    idx = range(pc.nparams) if params is None else params
+    g = np.zeros(pc.nparams)
+    for k in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += np.pi / 2
+        tm[k] -= np.pi / 2
+        g[k] = 0.5 * (expectation(pc, tp, terms) - expectation(pc, tm, terms))
+    return g
+
+
+def finite_difference(pc, theta, terms, h=1e-6, params=None) -> np.ndarray:
+    theta = np.asarray(theta, dtype=np.float64)
+    idx = range(pc.nparams) if params is None else params
+    g = np.zeros(pc.nparams)
+    for k in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        g[k] = (expectation(pc, tp, terms) - expectation(pc, tm, terms)) / (2 * h)
+    return g

@@ -91,8 +91,13 @@ SIGNATURES = {
     "qs_sv_apply_pauli_sum": (_I, [_P, _P, _P, _I, _I, _P, _I, _P]),
     "qs_sv_inner": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_adjoint_gradient": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _I, _P]),
-    "qs_qnn_batch": (_I, [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _P]),
+    "qs_qnn_batch": (_I, [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
 }
+
+
+class QsQnnConfig(ctypes.Structure):
+    _fields_ = [("lds_max_width", ctypes.c_int32), ("block_threads", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
 
 _lib = None
 _lib_lock = threading.Lock()

@@ -26,6 +26,8 @@
 #include "qsim_internal.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace qs {
@@ -762,6 +764,25 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
     if (rc) return rc;
     hdr_of[i] = (int)hdrs.size();
     hdrs.push_back(H);
+  }
+  if (const char* dbg = getenv("QS_FUSION_DEBUG")) {
+    if (dbg[0] == '1') {
+      int nrel = 0, kinds[8] = {0};
+      for (const FOp& o : ops) {
+        if (o.kind == F_RELAYOUT) ++nrel;
+        if (o.kind >= 0 && o.kind < 8) ++kinds[o.kind];
+      }
+      int nf = 0, lds = 0;
+      for (auto& h : hdrs) {
+        ++nf;
+        lds += h.uses_lds;
+      }
+      fprintf(stderr,
+              "[qsim fusion] n=%d K=%d L=%d gates=%d passes=%zu fused=%d with_lds=%d ops=%zu relayouts=%d "
+              "dense1=%d perm1=%d diag=%d dense2=%d swap2=%d dense3=%d dense4=%d\n",
+              n, K, L, ngates, plans.size(), nf, lds, ops.size(), nrel, kinds[F_DENSE1], kinds[F_PERM1],
+              kinds[F_DIAG], kinds[F_DENSE2], kinds[F_SWAP2], kinds[F_DENSE3], kinds[F_DENSE4]);
+    }
   }
   // upload program: [hdrs][ops][coef in R]
   const uint64_t hb = ((hdrs.size() * sizeof(PassHdr)) + 255) & ~255ull;

@@ -1,0 +1,431 @@
+// Batched variational circuits: one persistent launch runs forward + adjoint
+// backward for every circuit of a batch (no reference counterpart).
+//
+// Circuit c (width w = widths[c], qubit 0 = most significant amplitude bit):
+//   angle encoding   Ry(x[c, j]) on qubit j mod w, j < nfeat
+//   `layers` times   Ry(theta[l,0,q]) Rz(theta[l,1,q]) on every qubit q, then CX(q, q+1), q = 0..w-2
+//   readout          E_c = <Z_0>,  loss_c = (E_c - target_c)^2,  loss = mean_c loss_c
+//
+// Mechanism.  Ry rotations about one axis add, so the encoding is the product
+// state prod_q (cos(phi_q/2)|0> + sin(phi_q/2)|1>) with phi_q = sum_{j = q mod w} x_j:
+// one pass.  A CX ladder is the GF(2)-linear permutation "prefix xor" of the
+// qubit bits, so no amplitude is moved: the kernel keeps, per logical qubit q,
+// the physical index mask c_q = P(e_q) and the parity mask r_q (logical bit q of
+// a physical index p is parity(p & r_q)), and updates both after every ladder.
+// A rotation on logical qubit q then pairs physical p with p ^ c_q.  Every
+// circuit therefore costs w rotation passes per layer forward and w fused
+// (gradient + inverse) passes per layer backward, on a state that lives in LDS
+// (psi and lam, 2 x 2^w amplitudes) when w <= lds_max_width, otherwise in a
+// per-workgroup global slot.  Workgroups pull circuits from an atomic queue
+// ordered by decreasing width, so the widest circuits start first.
+#include "qsim_common.hpp"
+#include "qsim_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace qs {
+
+constexpr int QNN_MAX_W = 20;
+constexpr int QNN_MAX_LAYERS = 16;
+
+struct QnnArgs {
+  int ncirc;
+  int nfeat;
+  int layers;
+  int max_width;
+  int lds_max_width;
+  const int32_t* order;    // circuits, widest first
+  const int32_t* widths;
+  const double* x;         // ncirc x nfeat
+  const double* target;    // ncirc
+  const double* theta;     // layers x 2 x max_width
+  double* expval;          // ncirc
+  double* loss_terms;      // ncirc
+  double* grad_theta;      // layers x 2 x max_width (accumulated with atomics)
+  double* grad_x;          // ncirc x nfeat
+  void* slots;             // global per-workgroup state slots (2 x 2^max_width amplitudes each)
+  int* counter;
+  double inv_ncirc;
+};
+
+__device__ __forceinline__ int parity64(uint64_t v) { return __popcll(v) & 1; }
+
+template <typename R>
+__device__ __forceinline__ double block_reduce(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+template <typename R>
+__device__ __forceinline__ void block_reduce2(double& a, double& b, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * (threadIdx.x >> 6)] = a;
+    red[2 * (threadIdx.x >> 6) + 1] = b;
+  }
+  __syncthreads();
+  double sa = 0, sb = 0;
+  for (int i = 0; i < nw; ++i) {
+    sa += red[2 * i];
+    sb += red[2 * i + 1];
+  }
+  a = sa;
+  b = sb;
+}
+
+// Pair (p0, p1) of physical indices for logical qubit with column mask m and
+// row mask r; h enumerates 2^(w-1) pairs.  Returns p0 with logical bit 0.
+__device__ __forceinline__ void pair_of(uint64_t h, uint64_t m, uint64_t r, uint64_t& p0, uint64_t& p1) {
+  const int pivot = 63 - __clzll(m);
+  uint64_t a = ((h >> pivot) << (pivot + 1)) | (h & ((1ull << pivot) - 1));
+  uint64_t b = a ^ m;
+  if (parity64(a & r)) {
+    p0 = b;
+    p1 = a;
+  } else {
+    p0 = a;
+    p1 = b;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(1024) void k_qnn(QnnArgs A) {
+  extern __shared__ __align__(16) char lds_raw[];
+  __shared__ double red[64];
+  __shared__ uint64_t cm[QNN_MAX_W], rm[QNN_MAX_W];
+  __shared__ double gacc[QNN_MAX_LAYERS * 2 * QNN_MAX_W];
+  __shared__ double enc_c[QNN_MAX_W], enc_s[QNN_MAX_W];
+  __shared__ int cur;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  cx<R>* slot_psi = reinterpret_cast<cx<R>*>(A.slots) + (uint64_t)blockIdx.x * 2 * (1ull << A.max_width);
+  cx<R>* slot_lam = slot_psi + (1ull << A.max_width);
+
+  for (;;) {
+    if (tid == 0) cur = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int k = cur;
+    __syncthreads();
+    if (k >= A.ncirc) break;
+    const int c = A.order[k];
+    const int w = A.widths[c];
+    const uint64_t N = 1ull << w, half = N >> 1;
+    cx<R>* psi;
+    cx<R>* lam;
+    if (w <= A.lds_max_width) {
+      psi = reinterpret_cast<cx<R>*>(lds_raw);
+      lam = psi + N;
+    } else {
+      psi = slot_psi;
+      lam = slot_lam;
+    }
+    const double* th = A.theta;
+    const int MW = A.max_width;
+    // encoding angles and product state
+    for (int i = tid; i < 2 * A.layers * MW; i += nt) gacc[i] = 0;
+    if (tid < w) {
+      cm[tid] = 1ull << (w - 1 - tid);
+      rm[tid] = 1ull << (w - 1 - tid);
+      double phi = 0;
+      for (int j = tid; j < A.nfeat; j += w) phi += A.x[(uint64_t)c * A.nfeat + j];
+      enc_c[tid] = cos(0.5 * phi);
+      enc_s[tid] = sin(0.5 * phi);
+    }
+    __syncthreads();
+    for (uint64_t i = tid; i < N; i += nt) {
+      double re = 1;
+      for (int q = 0; q < w; ++q) re *= ((i >> (w - 1 - q)) & 1) ? enc_s[q] : enc_c[q];
+      st(psi + i, mk<R>((R)re, (R)0));
+    }
+    __syncthreads();
+    // forward
+    for (int l = 0; l < A.layers; ++l) {
+      for (int q = 0; q < w; ++q) {
+        const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
+        const double cy = cos(0.5 * ty), sy = sin(0.5 * ty), cz = cos(0.5 * tz), sz = sin(0.5 * tz);
+        // U = Rz(tz) Ry(ty) = [[e^{-i tz/2} cy, -e^{-i tz/2} sy], [e^{i tz/2} sy, e^{i tz/2} cy]]
+        const cx<R> u00 = mk<R>((R)(cz * cy), (R)(-sz * cy)), u01 = mk<R>((R)(-cz * sy), (R)(sz * sy));
+        const cx<R> u10 = mk<R>((R)(cz * sy), (R)(sz * sy)), u11 = mk<R>((R)(cz * cy), (R)(sz * cy));
+        const uint64_t m = cm[q], r = rm[q];
+        for (uint64_t h = tid; h < half; h += nt) {
+          uint64_t p0, p1;
+          pair_of(h, m, r, p0, p1);
+          const cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1);
+          st(psi + p0, cfma(mul(u00, a0), u01, a1));
+          st(psi + p1, cfma(mul(u10, a0), u11, a1));
+        }
+        __syncthreads();
+      }
+      // CX ladder: r'_k = r_0 ^ .. ^ r_k ; c'_q = c_q ^ c_{q+1}
+      if (tid == 0) {
+        for (int q = 1; q < w; ++q) rm[q] ^= rm[q - 1];
+        for (int q = 0; q + 1 < w; ++q) cm[q] ^= cm[q + 1];
+      }
+      __syncthreads();
+    }
+    // E = <Z_0>, lam = Z_0 psi
+    double e = 0;
+    {
+      const uint64_t r0 = rm[0];
+      for (uint64_t i = tid; i < N; i += nt) {
+        const cx<R> a = ld(psi + i);
+        const double p = (double)a.re * a.re + (double)a.im * a.im;
+        const bool one = parity64(i & r0);
+        e += one ? -p : p;
+        st(lam + i, one ? mk<R>(-a.re, -a.im) : a);
+      }
+    }
+    e = block_reduce<R>(e, red);
+    const double tgt = A.target[c];
+    const double dl = 2.0 * (e - tgt) * A.inv_ncirc;  // d(mean loss)/dE_c
+    // backward
+    for (int l = A.layers - 1; l >= 0; --l) {
+      if (tid == 0) {
+        for (int q = w - 1; q >= 1; --q) rm[q] ^= rm[q - 1];
+        for (int q = w - 2; q >= 0; --q) cm[q] ^= cm[q + 1];
+      }
+      __syncthreads();
+      for (int q = 0; q < w; ++q) {
+        const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
+        const double cy = cos(0.5 * ty), sy = sin(0.5 * ty), cz = cos(0.5 * tz), sz = sin(0.5 * tz);
+        const uint64_t m = cm[q], r = rm[q];
+        double gz = 0, gy = 0;
+        for (uint64_t h = tid; h < half; h += nt) {
+          uint64_t p0, p1;
+          pair_of(h, m, r, p0, p1);
+          cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1), b0 = ld(lam + p0), b1 = ld(lam + p1);
+          // d/dtz: Im <lam| Z |psi>
+          gz += ((double)b0.re * a0.im - (double)b0.im * a0.re) - ((double)b1.re * a1.im - (double)b1.im * a1.re);
+          // Rz^dagger = diag(e^{i tz/2}, e^{-i tz/2})
+          const cx<R> ep = mk<R>((R)cz, (R)sz), em = mk<R>((R)cz, (R)-sz);
+          a0 = mul(ep, a0);
+          b0 = mul(ep, b0);
+          a1 = mul(em, a1);
+          b1 = mul(em, b1);
+          // d/dty: Im <lam| Y |psi> with Y psi = (-i a1, i a0):
+          // Im(conj(b0)(-i a1)) = -Re(conj(b0) a1),  Im(conj(b1)(i a0)) = Re(conj(b1) a0)
+          gy += -((double)b0.re * a1.re + (double)b0.im * a1.im) + ((double)b1.re * a0.re + (double)b1.im * a0.im);
+          // Ry^dagger = [[cy, sy], [-sy, cy]]
+          const cx<R> n0 = mk<R>((R)(cy * a0.re + sy * a1.re), (R)(cy * a0.im + sy * a1.im));
+          const cx<R> n1 = mk<R>((R)(-sy * a0.re + cy * a1.re), (R)(-sy * a0.im + cy * a1.im));
+          const cx<R> l0 = mk<R>((R)(cy * b0.re + sy * b1.re), (R)(cy * b0.im + sy * b1.im));
+          const cx<R> l1 = mk<R>((R)(-sy * b0.re + cy * b1.re), (R)(-sy * b0.im + cy * b1.im));
+          st(psi + p0, n0);
+          st(psi + p1, n1);
+          st(lam + p0, l0);
+          st(lam + p1, l1);
+        }
+        block_reduce2<R>(gz, gy, red);
+        if (tid == 0) {
+          gacc[(l * 2 + 0) * MW + q] += gy;
+          gacc[(l * 2 + 1) * MW + q] += gz;
+        }
+        __syncthreads();
+      }
+    }
+    // encoding gradient: dE/dphi_q = Im <lam| Y_q |psi> (identity map again)
+    for (int q = 0; q < w; ++q) {
+      const uint64_t m = 1ull << (w - 1 - q);
+      double gy = 0, dummy = 0;
+      for (uint64_t h = tid; h < half; h += nt) {
+        uint64_t p0, p1;
+        pair_of(h, m, m, p0, p1);
+        const cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1), b0 = ld(lam + p0), b1 = ld(lam + p1);
+        gy += -((double)b0.re * a1.re + (double)b0.im * a1.im) + ((double)b1.re * a0.re + (double)b1.im * a0.im);
+      }
+      block_reduce2<R>(gy, dummy, red);
+      if (tid == 0)
+        for (int j = q; j < A.nfeat; j += w) A.grad_x[(uint64_t)c * A.nfeat + j] = dl * gy;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      A.expval[c] = e;
+      A.loss_terms[c] = (e - tgt) * (e - tgt);
+    }
+    for (int i = tid; i < A.layers * 2 * MW; i += nt) {
+      const int q = i % MW;
+      if (q < w && gacc[i] != 0.0) atomicAdd(A.grad_theta + i, dl * gacc[i]);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_qnn_clear(double* g, int ng, int* counter) {
+  for (int i = threadIdx.x; i < ng; i += blockDim.x) g[i] = 0;
+  if (threadIdx.x == 0) *counter = 0;
+}
+
+struct QnnPlan {
+  int dtype = -1, ncirc = 0, nfeat = 0, layers = 0, max_width = 0, lds_max_width = 0, threads = 0, blocks = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t stream = nullptr;
+  void* buf = nullptr;
+  uint64_t buf_bytes = 0;
+};
+
+static QnnPlan g_plan;  // one cached plan (graph + buffers) per process
+
+template <typename R>
+static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int nfeat, const double* x,
+                 const double* target, int layers, int max_width, const double* theta, double* expval, double* loss,
+                 double* grad_theta, double* grad_x, const qs_qnn_config* cfg, hipStream_t user_stream) {
+  int lds_w = cfg ? cfg->lds_max_width : 11;
+  int threads = cfg ? cfg->block_threads : 256;
+  int per_cu = cfg ? cfg->blocks_per_cu : 2;
+  const int use_graph = cfg ? cfg->use_graph : 1;
+  QS_REQUIRE(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
+             "block_threads must be a power of two in [64, 1024]");
+  QS_REQUIRE(lds_w >= 0 && lds_w <= 13, "lds_max_width must be in [0, 13]");
+  QS_REQUIRE(per_cu >= 1 && per_cu <= 8, "blocks_per_cu must be in [1, 8]");
+  const size_t lds_bytes = lds_w > 0 ? (size_t)2 * (1ull << lds_w) * sizeof(cx<R>) : 0;
+  QS_REQUIRE(lds_bytes <= 150 * 1024, "LDS state too large for one workgroup");
+  int dev = ctx->device, cus = 0;
+  QS_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = std::max(1, cus * per_cu);
+  // device buffers: order, widths, x, target, theta, expval, loss, grad_theta, grad_x, counter, slots
+  const uint64_t slot_bytes = (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>);
+  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+  const uint64_t nth = (uint64_t)layers * 2 * max_width;
+  const uint64_t sz_i = al(ncirc * sizeof(int32_t)), sz_x = al((uint64_t)ncirc * nfeat * sizeof(double)),
+                 sz_c = al(ncirc * sizeof(double)), sz_t = al(nth * sizeof(double));
+  const uint64_t total = 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + slot_bytes;
+  QnnPlan& P = g_plan;
+  const bool same = P.exec && P.dtype == dtype && P.ncirc == ncirc && P.nfeat == nfeat && P.layers == layers &&
+                    P.max_width == max_width && P.lds_max_width == lds_w && P.threads == threads &&
+                    P.blocks == blocks;
+  if (!same) {
+    if (P.exec) (void)hipGraphExecDestroy(P.exec);
+    if (P.graph) (void)hipGraphDestroy(P.graph);
+    P.exec = nullptr;
+    P.graph = nullptr;
+    if (!P.stream) QS_CHECK_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    if (P.buf_bytes < total) {
+      if (P.buf) QS_CHECK_HIP(hipFree(P.buf));
+      P.buf = nullptr;
+      QS_CHECK_HIP(hipMalloc(&P.buf, total));
+      P.buf_bytes = total;
+    }
+  }
+  char* b = (char*)P.buf;
+  int32_t* d_order = (int32_t*)b;
+  int32_t* d_w = (int32_t*)(b + sz_i);
+  double* d_x = (double*)(b + 2 * sz_i);
+  double* d_gx = (double*)(b + 2 * sz_i + sz_x);
+  double* d_t = (double*)(b + 2 * sz_i + 2 * sz_x);
+  double* d_e = (double*)(b + 2 * sz_i + 2 * sz_x + sz_c);
+  double* d_l = (double*)(b + 2 * sz_i + 2 * sz_x + 2 * sz_c);
+  double* d_th = (double*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c);
+  double* d_gth = (double*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c + sz_t);
+  int* d_cnt = (int*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t);
+  void* d_slots = b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256;
+
+  // widest first
+  std::vector<int32_t> order(ncirc);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c2) { return widths[a] > widths[c2]; });
+  hipStream_t s = P.stream;
+  QS_CHECK_HIP(hipMemcpyAsync(d_order, order.data(), ncirc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_w, widths, ncirc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_x, x, (uint64_t)ncirc * nfeat * sizeof(double), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_t, target, ncirc * sizeof(double), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_th, theta, nth * sizeof(double), hipMemcpyHostToDevice, s));
+
+  QnnArgs A;
+  A.ncirc = ncirc;
+  A.nfeat = nfeat;
+  A.layers = layers;
+  A.max_width = max_width;
+  A.lds_max_width = lds_w;
+  A.order = d_order;
+  A.widths = d_w;
+  A.x = d_x;
+  A.target = d_t;
+  A.theta = d_th;
+  A.expval = d_e;
+  A.loss_terms = d_l;
+  A.grad_theta = d_gth;
+  A.grad_x = d_gx;
+  A.slots = d_slots;
+  A.counter = d_cnt;
+  A.inv_ncirc = 1.0 / ncirc;
+  if (lds_bytes > 64 * 1024)
+    QS_CHECK_HIP(hipFuncSetAttribute((const void*)k_qnn<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_bytes));
+  int rc = prof_begin(ctx, PROF_QNN, s);
+  if (rc) return rc;
+  if (use_graph) {
+    if (!same) {
+      // clear + persistent kernel, captured once and replayed for every batch of this shape
+      QS_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(256), 0, s, d_gth, (int)nth, d_cnt);
+      hipLaunchKernelGGL(k_qnn<R>, dim3(blocks), dim3(threads), lds_bytes, s, A);
+      QS_CHECK_HIP(hipStreamEndCapture(s, &P.graph));
+      QS_CHECK_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+    }
+    QS_CHECK_HIP(hipGraphLaunch(P.exec, s));
+  } else {
+    hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(256), 0, s, d_gth, (int)nth, d_cnt);
+    QS_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_qnn<R>, dim3(blocks), dim3(threads), lds_bytes, s, A);
+    QS_CHECK_HIP(hipGetLastError());
+  }
+  rc = prof_end(ctx, s);
+  if (rc) return rc;
+  P.dtype = dtype;
+  P.ncirc = ncirc;
+  P.nfeat = nfeat;
+  P.layers = layers;
+  P.max_width = max_width;
+  P.lds_max_width = lds_w;
+  P.threads = threads;
+  P.blocks = blocks;
+  std::vector<double> lt(ncirc);
+  QS_CHECK_HIP(hipMemcpyAsync(expval, d_e, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
+  QS_CHECK_HIP(hipMemcpyAsync(lt.data(), d_l, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
+  QS_CHECK_HIP(hipMemcpyAsync(grad_theta, d_gth, nth * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (grad_x)
+    QS_CHECK_HIP(hipMemcpyAsync(grad_x, d_gx, (uint64_t)ncirc * nfeat * sizeof(double), hipMemcpyDeviceToHost, s));
+  QS_CHECK_HIP(hipStreamSynchronize(s));
+  (void)user_stream;
+  double L = 0;
+  for (int i = 0; i < ncirc; ++i) L += lt[i];
+  *loss = L / ncirc;
+  return QS_OK;
+}
+
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int nfeat, const double* x,
+                            const double* target, int layers, int max_width, const double* theta, double* expval,
+                            double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg,
+                            void* stream) {
+  QS_REQUIRE(ctx && widths && x && target && theta && expval && loss && grad_theta, "null argument");
+  QS_REQUIRE(dtype == QS_C64 || dtype == QS_C128, "dtype must be QS_C64 or QS_C128");
+  QS_REQUIRE(ncirc >= 1 && nfeat >= 0 && layers >= 1 && layers <= QNN_MAX_LAYERS, "bad batch shape");
+  QS_REQUIRE(max_width >= 1 && max_width <= QNN_MAX_W, "max_width must be in [1, 20]");
+  for (int i = 0; i < ncirc; ++i) QS_REQUIRE(widths[i] >= 2 && widths[i] <= max_width, "width out of range");
+  if (dtype == QS_C128)
+    return qnn_t<double>(ctx, dtype, ncirc, widths, nfeat, x, target, layers, max_width, theta, expval, loss,
+                         grad_theta, grad_x, cfg, as_stream(stream));
+  return qnn_t<float>(ctx, dtype, ncirc, widths, nfeat, x, target, layers, max_width, theta, expval, loss, grad_theta,
+                      grad_x, cfg, as_stream(stream));
+}

@@ -1,0 +1,301 @@
+"""Gradients of expectation values of parameterised circuits on MI355X.
+
+No reference counterpart (QuICT's QNN/VQE code is in the external QuICT_ml
+package).  Builds on the reference's gate definitions and inverse rules
+(QuICT/core/gate/gate.py: Rx/Ry/Rz/CRz/Rxx/Ryy/Rzz/Rzx are exp(-i theta P / 2),
+U1/Phase are diag(1, e^{i theta})).
+
+* ``adjoint_gradient``: forward simulation, lam = H psi, then one backward
+  sweep (qs_adjoint_gradient) that recomputes earlier states by applying gate
+  inverses — no per-layer checkpoints — and accumulates
+  dE/dtheta_k = 2 Re <lam| dG_k |psi_before_k>.
+* ``parameter_shift_gradient``: [E(theta + pi/2 e_k) - E(theta - pi/2 e_k)] / 2,
+  exact for exp(-i theta P / 2) gates; every evaluation is a GPU simulation.
+* ``finite_difference_gradient``: central differences (h = 1e-6), a coarse check.
+* ``AdjointExpectation``: torch.autograd.Function wrapping the adjoint path.
+"""
+import ctypes
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ..core import GateType, GATE_TYPE_TO_CLASS, Circuit
+from ..core import exception as qerr
+from .. import _native as nat
+from ..simulation.gate_compiler import compile_gate
+
+_PAULI = {
+    "I": np.eye(2, dtype=np.complex128),
+    "X": np.array([[0, 1], [1, 0]], dtype=np.complex128),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=np.complex128),
+    "Z": np.array([[1, 0], [0, -1]], dtype=np.complex128),
+}
+
+# generator P of G(theta) = exp(-i theta P / 2) on the gate's *target* qubits
+_ROTATION_GENERATORS = {
+    GateType.rx: _PAULI["X"], GateType.ry: _PAULI["Y"], GateType.rz: _PAULI["Z"], GateType.crz: _PAULI["Z"],
+    GateType.rxx: np.kron(_PAULI["X"], _PAULI["X"]), GateType.ryy: np.kron(_PAULI["Y"], _PAULI["Y"]),
+    GateType.rzz: np.kron(_PAULI["Z"], _PAULI["Z"]), GateType.rzx: np.kron(_PAULI["Z"], _PAULI["X"]),
+}
+_PHASE_TYPES = (GateType.u1, GateType.phase, GateType.cu1)
+SHIFT_RULE_TYPES = tuple(_ROTATION_GENERATORS)
+
+
+class PauliSum:
+    """sum_k c_k P_k with P_k a Pauli string over qubits 0..n-1 (qubit 0 first)."""
+
+    def __init__(self, n: int, terms: Sequence[Tuple[float, str]]):
+        self.n = n
+        self.terms = [(float(c), s.upper()) for c, s in terms]
+        for _, s in self.terms:
+            if len(s) != n or any(ch not in "IXYZ" for ch in s):
+                raise qerr.ValueError("Pauli strings must have one I/X/Y/Z letter per qubit")
+
+    def records(self) -> np.ndarray:
+        arr = (nat.QsPauliTerm * len(self.terms))()
+        for k, (c, s) in enumerate(self.terms):
+            x = z = 0
+            for q, ch in enumerate(s):
+                bit = 1 << (self.n - 1 - q)
+                if ch in "XY":
+                    x |= bit
+                if ch in "ZY":
+                    z |= bit
+            arr[k].coef, arr[k].xmask, arr[k].zmask = c, x, z
+        return arr
+
+    def matrix(self) -> np.ndarray:
+        d = 1 << self.n
+        m = np.zeros((d, d), dtype=np.complex128)
+        for c, s in self.terms:
+            t = np.array([[1.0 + 0j]])
+            for ch in s:
+                t = np.kron(t, _PAULI[ch])
+            m += c * t
+        return m
+
+
+class ParameterizedCircuit:
+    """Gate list whose angles come from a parameter vector.
+
+    ``add(GateType.ry, [q], param=k)`` binds the gate's single angle to
+    theta[k]; ``add(GateType.cx, [c, t])`` adds a fixed gate;
+    ``value=`` fixes the angle(s) of a parameterised type.
+    """
+
+    def __init__(self, n: int):
+        self.n = n
+        self.ops: List[Tuple[GateType, List[int], Optional[int], Optional[list]]] = []
+        self.nparams = 0
+
+    def add(self, gtype: GateType, qubits, param: Optional[int] = None, value=None):
+        qubits = [qubits] if isinstance(qubits, int) else list(qubits)
+        if param is not None:
+            if gtype not in _ROTATION_GENERATORS and gtype not in _PHASE_TYPES:
+                raise qerr.ValueError(f"{gtype.name} cannot carry a trainable angle")
+            self.nparams = max(self.nparams, param + 1)
+        vals = None if value is None else (list(value) if isinstance(value, (list, tuple)) else [value])
+        self.ops.append((gtype, qubits, param, vals))
+        return self
+
+    @staticmethod
+    def hea(n: int, layers: int) -> "ParameterizedCircuit":
+        """Ry + Rz on every qubit then a CX ladder, per layer; params ordered (layer, {Ry, Rz}, qubit)."""
+        pc = ParameterizedCircuit(n)
+        for l in range(layers):
+            for q in range(n):
+                pc.add(GateType.ry, [q], param=(l * 2 + 0) * n + q)
+                pc.add(GateType.rz, [q], param=(l * 2 + 1) * n + q)
+            for q in range(n - 1):
+                pc.add(GateType.cx, [q, q + 1])
+        return pc
+
+    def gates(self, theta: np.ndarray):
+        out = []
+        for gtype, qubits, param, vals in self.ops:
+            g = GATE_TYPE_TO_CLASS[gtype]()
+            if param is not None:
+                g = g(float(theta[param]))
+            elif vals is not None:
+                g = g(*vals)
+            c = g.controls
+            g.cargs = qubits[:c]
+            g.targs = qubits[c:]
+            out.append((g, param))
+        return out
+
+    def bind(self, theta: np.ndarray) -> Circuit:
+        c = Circuit(self.n)
+        for g, _ in self.gates(theta):
+            c.append(g)
+        return c
+
+
+def _derivative_target_block(g) -> np.ndarray:
+    """d G / d theta restricted to the target block (controls are kept as controls)."""
+    gt = g.type
+    t = g.targets
+    Gt = np.asarray(g.matrix, dtype=np.complex128)[-(1 << t):, -(1 << t):]
+    if gt in _ROTATION_GENERATORS:
+        return -0.5j * _ROTATION_GENERATORS[gt] @ Gt
+    if gt in _PHASE_TYPES:
+        D = np.zeros_like(Gt)
+        D[-1, -1] = 1j * Gt[-1, -1]
+        return D
+    raise qerr.ValueError(f"no derivative rule for {gt.name}")
+
+
+def _records(pc: ParameterizedCircuit, theta: np.ndarray):
+    n = pc.n
+    pairs = pc.gates(theta)
+    fwd = np.zeros(len(pairs), dtype=nat.QS_GATE_DTYPE)
+    inv = np.zeros(len(pairs), dtype=nat.QS_GATE_DTYPE)
+    der = np.zeros(len(pairs), dtype=nat.QS_GATE_DTYPE)
+    slot = np.full(len(pairs), -1, dtype=np.int32)
+    keep: list = []
+    for j, (g, param) in enumerate(pairs):
+        compile_gate(g, n, fwd[j], keep)
+        compile_gate(g, n, inv[j], keep, dagger=True)
+        if param is not None:
+            slot[j] = param
+            D = _derivative_target_block(g)
+            rec = der[j]
+            rec["kind"] = nat.GATE_DENSE
+            rec["ncontrols"] = g.controls
+            rec["ntargets"] = g.targets
+            args = list(g.cargs) + list(g.targs)
+            rec["bits"][: len(args)] = [n - 1 - a for a in args]
+            v = D.ravel()
+            rec["m"][: 2 * len(v)] = np.column_stack([v.real, v.imag]).ravel()
+            # the inverse record must carry the same control / target split
+            if inv[j]["ncontrols"] != g.controls or inv[j]["ntargets"] != g.targets:
+                raise qerr.GateAlgorithmNotImplementError(f"{g.type.name}: unexpected record shape")
+    return fwd, inv, der, slot, keep
+
+
+class _Engine:
+    def __init__(self, n: int, precision: str, device: int):
+        import torch
+        self.torch = torch
+        self.n = n
+        self.device = device
+        self.tdtype = torch.complex128 if precision == "double" else torch.complex64
+        self.dcode = nat.QS_C128 if precision == "double" else nat.QS_C64
+        self.lib = nat.load_library()
+        self.ctx = nat.context(device)
+        self.psi = torch.empty(1 << n, dtype=self.tdtype, device=f"cuda:{device}")
+        self.lam = torch.empty_like(self.psi)
+
+    def ptr(self, t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def stream(self):
+        return nat.stream_ptr(self.device)
+
+    def forward(self, fwd: np.ndarray, fuse: int = 1):
+        nat.check(self.lib.qs_sv_init(self.ctx, self.ptr(self.psi), self.n, self.dcode, 0, self.stream()), "init")
+        nat.check(self.lib.qs_sv_apply_gates(self.ctx, self.ptr(self.psi), self.n, self.dcode,
+                                             nat.gate_array_ptr(fwd), len(fwd), fuse, None, self.stream()), "forward")
+
+    def expectation(self, H: PauliSum) -> float:
+        recs = H.records()
+        nat.check(self.lib.qs_sv_apply_pauli_sum(self.ctx, self.ptr(self.psi), self.ptr(self.lam), self.n,
+                                                 self.dcode, ctypes.cast(recs, ctypes.c_void_p), len(recs),
+                                                 self.stream()), "pauli sum")
+        re, im = ctypes.c_double(0), ctypes.c_double(0)
+        nat.check(self.lib.qs_sv_inner(self.ctx, self.ptr(self.psi), self.ptr(self.lam), self.n, self.dcode,
+                                       ctypes.byref(re), ctypes.byref(im), self.stream()), "inner")
+        return re.value
+
+
+_engines = {}
+
+
+def _engine(n, precision, device) -> _Engine:
+    key = (n, precision, device)
+    if key not in _engines:
+        _engines[key] = _Engine(n, precision, device)
+    return _engines[key]
+
+
+def expectation(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, precision: str = "double",
+                device: int = 0) -> float:
+    eng = _engine(pc.n, precision, device)
+    fwd, _, _, _, keep = _records(pc, np.asarray(theta, dtype=np.float64))
+    eng.forward(fwd)
+    return eng.expectation(H)
+
+
+def adjoint_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, precision: str = "double",
+                     device: int = 0) -> Tuple[float, np.ndarray]:
+    theta = np.asarray(theta, dtype=np.float64)
+    eng = _engine(pc.n, precision, device)
+    fwd, inv, der, slot, keep = _records(pc, theta)
+    eng.forward(fwd)
+    E = eng.expectation(H)
+    grads = np.zeros(max(pc.nparams, 1), dtype=np.float64)
+    nat.check(eng.lib.qs_adjoint_gradient(eng.ctx, eng.ptr(eng.psi), eng.ptr(eng.lam), pc.n, eng.dcode,
+                                          nat.gate_array_ptr(inv), len(inv), nat.gate_array_ptr(der),
+                                          slot.ctypes.data_as(ctypes.c_void_p),
+                                          grads.ctypes.data_as(ctypes.c_void_p), len(grads), eng.stream()),
+              "qs_adjoint_gradient")
+    return E, grads[: pc.nparams]
+
+
+def parameter_shift_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, precision: str = "double",
+                             device: int = 0, params: Optional[Sequence[int]] = None) -> np.ndarray:
+    theta = np.asarray(theta, dtype=np.float64)
+    for gtype, _, p, _ in pc.ops:
+        if p is not None and gtype not in SHIFT_RULE_TYPES:
+            raise qerr.ValueError(f"the two-term shift rule does not hold for {gtype.name}")
+    idx = range(pc.nparams) if params is None else params
+    g = np.zeros(pc.nparams)
+    for k in idx:
+        tp = theta.copy()
+        tp[k] += np.pi / 2
+        tm = theta.copy()
+        tm[k] -= np.pi / 2
+        g[k] = 0.5 * (expectation(pc, tp, H, precision, device) - expectation(pc, tm, H, precision, device))
+    return g
+
+
+def finite_difference_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, h: float = 1e-6,
+                               precision: str = "double", device: int = 0,
+                               params: Optional[Sequence[int]] = None) -> np.ndarray:
+    theta = np.asarray(theta, dtype=np.float64)
+    idx = range(pc.nparams) if params is None else params
+    g = np.zeros(pc.nparams)
+    for k in idx:
+        tp = theta.copy()
+        tp[k] += h
+        tm = theta.copy()
+        tm[k] -= h
+        g[k] = (expectation(pc, tp, H, precision, device) - expectation(pc, tm, H, precision, device)) / (2 * h)
+    return g
+
+
+def _autograd_function():
+    import torch
+
+    class AdjointExpectation(torch.autograd.Function):
+        """E(theta) = <psi(theta)|H|psi(theta)>; backward by the adjoint method."""
+
+        @staticmethod
+        def forward(ctx, theta, pc, H, precision="double"):
+            E, g = adjoint_gradient(pc, theta.detach().cpu().numpy(), H, precision, theta.device.index or 0
+                                    if theta.is_cuda else 0)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=theta.dtype, device=theta.device))
+            return torch.as_tensor(E, dtype=theta.dtype, device=theta.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out * g, None, None, None
+
+    return AdjointExpectation
+
+
+def __getattr__(name):
+    if name == "AdjointExpectation":
+        return _autograd_function()
+    raise AttributeError(name)

@@ -1,0 +1,108 @@
+"""Batched QNN launch on the GPU against per-circuit references.
+
+Reference per circuit: the same circuit built gate by gate (16 Ry encodings,
+HEA layers) evaluated by the CPU oracle (small widths: <Z_0> and the
+parameter-shift gradient) or by the single-circuit GPU adjoint path
+(larger widths), then combined into the mean loss and its gradient."""
+import numpy as np
+import pytest
+
+from quict_amd import workloads
+from oracle import gradients as og
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def reference(widths, x, targets, theta, layers, max_width, use_gpu_adjoint=False):
+    from quict_amd.gradient.qnn import circuit_for
+    from quict_amd.gradient.adjoint import PauliSum, adjoint_gradient
+    N = len(widths)
+    ev = np.zeros(N)
+    g = np.zeros(layers * 2 * max_width)
+    for c in range(N):
+        w = int(widths[c])
+        pc = circuit_for(w, x[c], layers, max_width)
+        zterm = [(1.0, "Z" + "I" * (w - 1))]
+        if use_gpu_adjoint:
+            e, gc = adjoint_gradient(pc, theta, PauliSum(w, zterm))
+        else:
+            e = og.expectation(pc, theta, zterm)
+            gc = og.parameter_shift(pc, theta, zterm,
+                                    params=[(l * 2 + k) * max_width + q for l in range(layers) for k in range(2)
+                                            for q in range(w)])
+        ev[c] = e
+        g += 2 * (e - targets[c]) / N * gc
+    loss = np.mean((ev - targets) ** 2)
+    return loss, ev, g
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_small_batch_vs_cpu_shift(use_graph):
+    say("qnn small batch graph", use_graph)
+    from quict_amd.gradient.qnn import QNNBatch
+    widths = np.array([3, 5, 4, 6, 2, 5], dtype=np.int32)
+    rng = np.random.RandomState(1)
+    x = rng.randn(6, 7)
+    t = 1 - 2 * (x[:, 0] > 0)
+    layers, mw = 2, 6
+    theta = rng.uniform(-np.pi, np.pi, layers * 2 * mw)
+    qb = QNNBatch(widths, x, t, layers=layers, max_width=mw,
+                  config={"lds_max_width": 4, "block_threads": 128, "blocks_per_cu": 1, "use_graph": use_graph})
+    loss, ev, gth, gx = qb.forward_backward(theta)
+    rl, rev, rg = reference(widths, x, t, theta, layers, mw)
+    assert np.max(np.abs(ev - rev)) < 1e-12
+    assert abs(loss - rl) < 1e-12
+    assert np.max(np.abs(gth - rg)) < 1e-10
+    # second call reuses the captured graph with new inputs
+    theta2 = theta * 0.5
+    loss2, ev2, g2, _ = qb.forward_backward(theta2)
+    rl2, rev2, rg2 = reference(widths, x, t, theta2, layers, mw)
+    assert np.max(np.abs(ev2 - rev2)) < 1e-12 and np.max(np.abs(g2 - rg2)) < 1e-10
+
+
+def test_grad_x_by_finite_differences():
+    say("qnn grad x")
+    from quict_amd.gradient.qnn import QNNBatch
+    widths = np.array([4, 3], dtype=np.int32)
+    rng = np.random.RandomState(2)
+    x = rng.randn(2, 6)
+    t = np.array([1.0, -1.0])
+    theta = rng.uniform(-np.pi, np.pi, 2 * 2 * 4)
+    qb = QNNBatch(widths, x, t, layers=2, max_width=4)
+    loss, ev, gth, gx = qb.forward_backward(theta)
+    h = 1e-6
+    for c, j in [(0, 0), (0, 5), (1, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[c, j] += h
+        xm[c, j] -= h
+        lp = QNNBatch(widths, xp, t, layers=2, max_width=4).forward_backward(theta)[0]
+        lm = QNNBatch(widths, xm, t, layers=2, max_width=4).forward_backward(theta)[0]
+        assert abs(gx[c, j] - (lp - lm) / (2 * h)) < 1e-7
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_config_batch_sample_vs_gpu_adjoint(precision):
+    say("qnn config sample", precision)
+    from quict_amd.gradient.qnn import QNNBatch, synthetic_batch
+    widths, x, t = synthetic_batch(1024, 16, 8, 18, 4)
+    layers, mw = 6, 18
+    theta = np.random.RandomState(7).uniform(-np.pi, np.pi, layers * 2 * mw)
+    qb = QNNBatch(widths, x, t, layers=layers, max_width=mw, precision=precision)
+    loss, ev, gth, gx = qb.forward_backward(theta)
+    # check a width-spanning subset of circuits against the single-circuit GPU adjoint path
+    idx = [int(np.where(widths == w)[0][0]) for w in range(8, 19)]
+    sub_w, sub_x, sub_t = widths[idx], x[idx], t[idx]
+    sb = QNNBatch(sub_w, sub_x, sub_t, layers=layers, max_width=mw, precision=precision)
+    sl, sev, sg, _ = sb.forward_backward(theta)
+    rl, rev, rg = reference(sub_w, sub_x, sub_t, theta, layers, mw, use_gpu_adjoint=True)
+    tol_e, tol_g = (1e-12, 1e-10) if precision == "double" else (1e-4, 1e-4)
+    err_e = np.max(np.abs(sev - rev))
+    err_g = np.max(np.abs(sg - rg))
+    print(f"{precision}: max |E - ref| = {err_e:.2e}, max |grad - ref| = {err_g:.2e}")
+    assert err_e < tol_e and err_g < tol_g
+    assert np.max(np.abs(ev[idx] - sev)) < (1e-12 if precision == "double" else 1e-5)
+    assert np.isfinite(loss) and np.all(np.isfinite(gth))

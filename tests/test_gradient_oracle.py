@@ -1,0 +1,34 @@
+"""Pins the gradient oracle on the CPU: closed forms, shift rule vs finite differences."""
+import numpy as np
+
+from quict_amd.core import GateType
+from quict_amd.gradient.adjoint import ParameterizedCircuit, PauliSum
+from quict_amd import workloads
+from oracle import gradients as og
+
+
+def test_closed_form_single_qubit():
+    pc = ParameterizedCircuit(1).add(GateType.ry, [0], param=0)
+    for th in [0.3, 1.7, -2.2]:
+        assert abs(og.expectation(pc, [th], [(1.0, "Z")]) - np.cos(th)) < 1e-14
+        assert abs(og.parameter_shift(pc, [th], [(1.0, "Z")])[0] + np.sin(th)) < 1e-14
+        assert abs(og.finite_difference(pc, [th], [(1.0, "Z")])[0] + np.sin(th)) < 1e-8
+
+
+def test_pauli_apply_matches_dense_matrix():
+    n = 4
+    terms = workloads.zz_x_hamiltonian(n) + [(0.3, "YXZI"), (-0.7, "IYYI")]
+    H = PauliSum(n, terms).matrix()
+    rng = np.random.RandomState(0)
+    v = rng.randn(16) + 1j * rng.randn(16)
+    assert np.allclose(og.pauli_apply(v, n, terms), H @ v, atol=1e-13)
+
+
+def test_shift_rule_vs_finite_differences_hea():
+    n, layers = 4, 2
+    pc = ParameterizedCircuit.hea(n, layers)
+    th = workloads.hea_params(n, layers, 3).ravel()
+    terms = workloads.zz_x_hamiltonian(n)
+    ps = og.parameter_shift(pc, th, terms)
+    fd = og.finite_difference(pc, th, terms)
+    assert np.max(np.abs(ps - fd)) < 1e-7
