@@ -80,6 +80,12 @@ uint64_t qs_ctx_scratch_bytes(const qs_ctx* ctx);
 int qs_profile_enable(qs_ctx* ctx, int on);
 int qs_profile_read(qs_ctx* ctx, double* ms /*[4]*/, int64_t* launches /*[4]*/, int reset);
 
+/* Streaming copy of `bytes` (a multiple of 16) from src to dst, 16 B per lane,
+   optionally with non-temporal loads/stores: the achievable-bandwidth
+   reference that roofline figures are compared with (no reference
+   counterpart). */
+int qs_stream_copy(qs_ctx* ctx, void* dst, const void* src, uint64_t bytes, int nontemporal, void* stream);
+
 /* ---- state vector ------------------------------------------------------- */
 /* |index> : replaces StateVectorSimulator.initial_state_vector
    (QuICT/simulation/state_vector/statevector_simulator.py:122-129). */
@@ -103,6 +109,11 @@ int qs_sv_apply_gates(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 /* Fusion tile shape: log2 amplitudes per tile (0 = built-in default) and the
    number of low bits always kept contiguous.  Set by the autotuner. */
 int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
+
+/* Launch shape of the fused tile pass: prefetch != 0 selects persistent
+   workgroups (wg_per_cu per CU) that load the next tile while computing the
+   current one; prefetch == 0 launches one workgroup per tile. */
+int qs_set_fusion_launch(qs_ctx* ctx, int prefetch, int wg_per_cu);
 
 /* P(bit == 0) = sum |a|^2 over the half with that bit clear, reduced in double.
    Replaces measured_prob_calculate (gate_function.py:1751-1776). */

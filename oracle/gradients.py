@@ -37,15 +37,33 @@ def expectation(pc, theta, terms) -> float:
     return float(np.real(np.vdot(v, pauli_apply(v, pc.n, terms))))
 
 
+def _param_types(pc):
+    return {p: gt for gt, _, p, _ in pc.ops if p is not None}
+
+
 def parameter_shift(pc, theta, terms, params=None) -> np.ndarray:
+    """Two-term rule [E(t+pi/2) - E(t-pi/2)]/2 for exp(-i t P/2) gates; for
+    controlled rotations (generator eigenvalues {0, 0, +-1/2}) the four-term rule
+    c+ [E(t+pi/2) - E(t-pi/2)] - c- [E(t+3pi/2) - E(t-3pi/2)],
+    c+- = (sqrt2 +- 1) / (4 sqrt2)."""
+    from quict_amd.core import GateType
     theta = np.asarray(theta, dtype=np.float64)
     idx = range(pc.nparams) if params is None else params
+    types = _param_types(pc)
     g = np.zeros(pc.nparams)
+
+    def shifted(k, s):
+        t = theta.copy()
+        t[k] += s
+        return expectation(pc, t, terms)
+
+    cp, cm = (np.sqrt(2) + 1) / (4 * np.sqrt(2)), (np.sqrt(2) - 1) / (4 * np.sqrt(2))
     for k in idx:
-        tp, tm = theta.copy(), theta.copy()
-        tp[k] += np.pi / 2
-        tm[k] -= np.pi / 2
-        g[k] = 0.5 * (expectation(pc, tp, terms) - expectation(pc, tm, terms))
+        if types.get(k) == GateType.crz:
+            g[k] = cp * (shifted(k, np.pi / 2) - shifted(k, -np.pi / 2)) - \
+                cm * (shifted(k, 3 * np.pi / 2) - shifted(k, -3 * np.pi / 2))
+        else:
+            g[k] = 0.5 * (shifted(k, np.pi / 2) - shifted(k, -np.pi / 2))
     return g
 
 

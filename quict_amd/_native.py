@@ -71,8 +71,10 @@ SIGNATURES = {
     "qs_last_error": (ctypes.c_char_p, []),
     "qs_ctx_scratch_bytes": (_U64, [_P]),
     "qs_set_fusion": (_I, [_P, _I, _I]),
+    "qs_set_fusion_launch": (_I, [_P, _I, _I]),
     "qs_profile_enable": (_I, [_P, _I]),
     "qs_profile_read": (_I, [_P, _P, _P, _I]),
+    "qs_stream_copy": (_I, [_P, _P, _P, _U64, _I, _P]),
     "qs_sv_init": (_I, [_P, _P, _I, _I, _U64, _P]),
     "qs_sv_apply_gate": (_I, [_P, _P, _I, _I, _P, _P]),
     "qs_sv_apply_gates": (_I, [_P, _P, _I, _I, _P, _I, _I, ctypes.POINTER(_I), _P]),

@@ -74,6 +74,23 @@ int prof_collect(qs_ctx* ctx) {
   return QS_OK;
 }
 
+template <bool NT>
+__global__ __launch_bounds__(256) void k_copy16(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if constexpr (NT) {
+      double2 v;
+      v.x = __builtin_nontemporal_load(&src[i].x);
+      v.y = __builtin_nontemporal_load(&src[i].y);
+      __builtin_nontemporal_store(v.x, &dst[i].x);
+      __builtin_nontemporal_store(v.y, &dst[i].y);
+    } else {
+      dst[i] = src[i];
+    }
+  }
+}
+
 template <typename R> __global__ void k_init(cx<R>* __restrict__ v, uint64_t N, uint64_t index) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride)
@@ -264,6 +281,9 @@ qs_ctx* qs_ctx_create(int device) {
   }
   qs_ctx* ctx = new qs_ctx();
   ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->num_cus = cus;
   if (hipHostMalloc((void**)&ctx->host_pinned, 4096, hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     delete ctx;
@@ -315,6 +335,30 @@ int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits) {
   QS_REQUIRE(low_bits == 0 || (low_bits >= 2 && low_bits <= 8), "low_bits must be 0 or in [2, 8]");
   ctx->tile_log2 = tile_log2;
   ctx->low_bits = low_bits;
+  return QS_OK;
+}
+
+int qs_stream_copy(qs_ctx* ctx, void* dst, const void* src, uint64_t bytes, int nontemporal, void* stream) {
+  QS_REQUIRE(ctx && dst && src && bytes % 16 == 0, "bad copy arguments (bytes must be a multiple of 16)");
+  const uint64_t n16 = bytes / 16;
+  const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, 1ull << 20);
+  int rc = prof_begin(ctx, PROF_GATE, as_stream(stream));
+  if (rc) return rc;
+  if (nontemporal)
+    hipLaunchKernelGGL(k_copy16<true>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (const double2*)src,
+                       (double2*)dst, n16);
+  else
+    hipLaunchKernelGGL(k_copy16<false>, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), (const double2*)src,
+                       (double2*)dst, n16);
+  QS_CHECK_HIP(hipGetLastError());
+  return prof_end(ctx, as_stream(stream));
+}
+
+int qs_set_fusion_launch(qs_ctx* ctx, int prefetch, int wg_per_cu) {
+  QS_REQUIRE(ctx, "null ctx");
+  QS_REQUIRE(wg_per_cu >= 1 && wg_per_cu <= 8, "wg_per_cu must be in [1, 8]");
+  ctx->fused_prefetch = prefetch ? 1 : 0;
+  ctx->fused_wg_per_cu = wg_per_cu;
   return QS_OK;
 }
 

@@ -100,6 +100,9 @@ struct qs_ctx {
   int device = 0;
   int tile_log2 = 0;   // 0 = default per dtype
   int low_bits = 0;    // 0 = default
+  int num_cus = 256;   // compute units of the device
+  int fused_wg_per_cu = 2;  // resident workgroups per CU for the persistent tile kernel
+  int fused_prefetch = 0;   // 1: persistent prefetching tile kernel, 0: one tile per workgroup
   // device scratch, grown on demand
   void* scratch = nullptr;
   uint64_t scratch_bytes = 0;

@@ -45,6 +45,8 @@ enum FKind : int32_t {
 
 constexpr int FMAX_TB = 10;
 constexpr int FMAX_RB = 5;
+// Complex coefficients one pass may carry (staged in LDS next to the relayout buffer).
+constexpr int FMAX_PASS_COEF = 1024;
 
 struct FOp {
   int32_t kind;
@@ -65,6 +67,8 @@ struct PassHdr {
   int32_t op0;
   int32_t coef0;
   int32_t uses_lds;
+  int32_t has_dk;        // pass contains 3- or 4-target dense ops (kernel variant without prefetch)
+  int32_t ncoef;         // complex coefficients of this pass (staged in LDS by the prefetching variant)
   int32_t tbits[16];     // tile-local bit -> global bit
   int32_t sorted[16];    // tile bits ascending (global), for the tile base
   int32_t rs0[FMAX_RB], ts0[FMAX_TB];  // load layout (tile-local bits)
@@ -91,7 +95,19 @@ template <int RB> __device__ __forceinline__ uint32_t reg_local(int r, const int
   return j;
 }
 
-template <typename R> __device__ __forceinline__ cx<R> cload(const cx<R>* p) { return *p; }
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS (and scalar)
+// operations, not for outstanding global loads, so prefetched tile loads stay
+// in flight across relayouts.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Non-temporal amplitude loads / stores for the streaming tile traffic.
+template <typename R> __device__ __forceinline__ cx<R> ld_nt(const cx<R>* p) {
+  return cx<R>{__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+}
+template <typename R> __device__ __forceinline__ void st_nt(cx<R>* p, cx<R> a) {
+  __builtin_nontemporal_store(a.re, &p->re);
+  __builtin_nontemporal_store(a.im, &p->im);
+}
 
 template <typename R, int NR, int P>
 __device__ __forceinline__ void do_dense1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, bool tok) {
@@ -142,9 +158,6 @@ __device__ __forceinline__ void do_perm1(cx<R> (&a)[NR], const cx<R>* m, uint32_
 // positions P0 (MSB) > P1 (LSB)
 template <typename R, int NR, int P0, int P1>
 __device__ __forceinline__ void do_dense2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, bool tok) {
-  cx<R> mm[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) mm[i] = m[i];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (r & ((1 << P0) | (1 << P1))) continue;
@@ -156,10 +169,12 @@ __device__ __forceinline__ void do_dense2(cx<R> (&a)[NR], const cx<R>* m, uint32
       for (int j = 0; j < 4; ++j) x[j] = a[idx[j]];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        cx<R> acc = mul(mm[4 * i], x[0]);
+        // one matrix row at a time keeps scalar-register pressure low
+        cx<R> acc = mul(m[4 * i], x[0]);
 #pragma unroll
-        for (int j = 1; j < 4; ++j) acc = cfma(acc, mm[4 * i + j], x[j]);
+        for (int j = 1; j < 4; ++j) acc = cfma(acc, m[4 * i + j], x[j]);
         a[idx[i]] = acc;
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
@@ -216,81 +231,112 @@ __device__ __forceinline__ void do_densek(cx<R> (&a)[NR], const cx<R>* m, uint32
 #pragma unroll
         for (int j = 1; j < D; ++j) acc = cfma(acc, m[i * D + j], x[j]);
         a[r | densek_off<PM, KB>(i)] = acc;
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
   }
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ void do_diag(cx<R> (&a)[NR], const FOp& op, const cx<R>* d, uint32_t tid, bool tok) {
-  const int nr = op.npos, nt = op.ntt;
-  uint32_t tsel = 0;
-  for (int k = 0; k < nt; ++k) tsel = (tsel << 1) | ((tid >> op.tpos[k]) & 1u);
-  const uint32_t creg = op.creg;
+// Diagonal op: the host resolved every register-bit dependence at plan time, so
+// the payload is a table T[sel * NR + r] (sel = up to two thread-index bits, MSB
+// first) and `creg` is the mask of registers whose entry is not 1 for some sel.
+template <typename R, int NR, int NS>
+__device__ __forceinline__ void do_diag_t(cx<R> (&a)[NR], const cx<R>* T, uint32_t rmask, uint32_t sel) {
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    if (((uint32_t)r & creg) != creg) continue;
-    uint32_t ridx = 0;
-    for (int k = 0; k < nr; ++k) ridx = (ridx << 1) | (((uint32_t)r >> op.pos[k]) & 1u);
-    const cx<R>* e = d + (ridx << nt);
+    if (!((rmask >> r) & 1u)) continue;
     cx<R> val;
-    if (nt == 0) {
-      val = e[0];
-    } else if (nt == 1) {
-      const cx<R> e0 = e[0], e1 = e[1];
-      val = tsel ? e1 : e0;
+    if constexpr (NS == 0) {
+      val = T[r];
+    } else if constexpr (NS == 1) {
+      const cx<R> e0 = T[r], e1 = T[NR + r];
+      val = sel ? e1 : e0;
     } else {
-      const cx<R> e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
-      val = (tsel & 2) ? ((tsel & 1) ? e3 : e2) : ((tsel & 1) ? e1 : e0);
+      const cx<R> e0 = T[r], e1 = T[NR + r], e2 = T[2 * NR + r], e3 = T[3 * NR + r];
+      val = (sel & 2) ? ((sel & 1) ? e3 : e2) : ((sel & 1) ? e1 : e0);
     }
-    if (tok) a[r] = mul(val, a[r]);
+    a[r] = mul(val, a[r]);
   }
 }
 
-template <typename R, int RB, int TB>
-__global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const PassHdr* __restrict__ H,
-                                                   const FOp* __restrict__ ops, const cx<R>* __restrict__ coef) {
-  constexpr int NR = 1 << RB;
-  constexpr int K = RB + TB;
-  extern __shared__ __align__(16) char lds_raw[];
-  cx<R>* lds = reinterpret_cast<cx<R>*>(lds_raw);
-  const uint32_t tid = threadIdx.x;
+// Diagonal gate that depends on thread bits only: its value is a per-lane
+// scalar, which commutes with every in-register op (those mix registers of one
+// lane), so it is folded into the lane factor `f` and applied to the registers
+// only before a relayout or the store.
+template <typename R>
+__device__ __forceinline__ void diag_lane(cx<R>& f, const FOp& op, const cx<R>* T, uint32_t tid) {
+  const int ns = op.ntt;
+  uint32_t sel = 0;
+  for (int k = 0; k < ns; ++k) sel = (sel << 1) | ((tid >> op.tpos[k]) & 1u);
+  // component-wise selects: selecting whole structs by a lane value makes the
+  // compiler index a private (scratch) array
+  R re = T[0].re, im = T[0].im;
+  if (ns >= 1) {
+    const bool b0 = sel & 1u;
+    if (ns == 1) {
+      re = b0 ? T[1].re : re;
+      im = b0 ? T[1].im : im;
+    } else {
+      const bool b1 = sel & 2u;
+      const R lo_re = b0 ? T[1].re : re, lo_im = b0 ? T[1].im : im;
+      const R hi_re = b0 ? T[3].re : T[2].re, hi_im = b0 ? T[3].im : T[2].im;
+      re = b1 ? hi_re : lo_re;
+      im = b1 ? hi_im : lo_im;
+    }
+  }
+  f = mul(mk<R>(re, im), f);
+}
 
-  const uint64_t tbase = insert_zeros((uint64_t)blockIdx.x, H->sorted, K);
+template <typename R, int NR>
+__device__ __forceinline__ void flush_lane(cx<R> (&a)[NR], cx<R>& f) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) a[r] = mul(f, a[r]);
+  f = mk<R>((R)1, (R)0);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void do_diag(cx<R> (&a)[NR], const FOp& op, const cx<R>* T, uint32_t tid) {
+  const int ns = op.ntt;
+  uint32_t sel = 0;
+  for (int k = 0; k < ns; ++k) sel = (sel << 1) | ((tid >> op.tpos[k]) & 1u);
+  // general table: per-lane loads from row `sel` (the address is lane-dependent,
+  // so the 16..64 table entries never occupy scalar registers)
+  const cx<R>* Tl = T + sel * NR;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    if ((op.creg >> r) & 1u) a[r] = mul(ld(Tl + r), a[r]);
+}
+
+// Apply a pass's op list to the register tile `a` (load layout on entry, store
+// layout on exit).  DK: include the in-register 3- and 4-target dense ops.
+template <typename R, int RB, int TB, bool DK>
+__device__ __forceinline__ void run_ops(cx<R> (&a)[1 << RB], cx<R>* lds, const PassHdr* __restrict__ H,
+                                        const FOp* __restrict__ P, const cx<R>* __restrict__ C, uint32_t tid) {
+  constexpr int NR = 1 << RB;
   int rs[RB];
 #pragma unroll
   for (int b = 0; b < RB; ++b) rs[b] = H->rs0[b];
-  uint64_t gthr = 0;
   uint32_t thr_loc = 0;
 #pragma unroll
-  for (int b = 0; b < TB; ++b) {
-    const uint32_t bit = (tid >> b) & 1u;
-    gthr |= (uint64_t)bit << H->tbits[H->ts0[b]];
-    thr_loc |= bit << H->ts0[b];
-  }
-  cx<R> a[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    uint64_t g = 0;
-#pragma unroll
-    for (int b = 0; b < RB; ++b) g |= (uint64_t)((r >> b) & 1) << H->tbits[rs[b]];
-    a[r] = ld(v + (tbase | gthr | g));
-  }
-
+  for (int b = 0; b < TB; ++b) thr_loc |= ((tid >> b) & 1u) << H->ts0[b];
   const int nops = H->nops;
-  const FOp* P = ops + H->op0;
-  const cx<R>* C = coef + H->coef0;
+  cx<R> f = mk<R>((R)1, (R)0);  // deferred per-lane diagonal factor
+  bool fdirty = false;
   for (int o = 0; o < nops; ++o) {
     const FOp& op = P[o];
     const int kind = op.kind;
     if (kind == F_RELAYOUT) {
-      __syncthreads();
+      if (fdirty) {
+        flush_lane<R, NR>(a, f);
+        fdirty = false;
+      }
+      lds_barrier();
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         const uint32_t j = thr_loc | reg_local<RB>(r, rs);
         lds[lds_swz(j)] = a[r];
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int b = 0; b < RB; ++b) rs[b] = op.rs[b];
       thr_loc = 0;
@@ -327,7 +373,12 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         }
         break;
       case F_DIAG:
-        do_diag<R, NR>(a, op, m, tid, tok);
+        if (op.pure) {
+          diag_lane<R>(f, op, m, tid);
+          fdirty = true;
+        } else {
+          do_diag<R, NR>(a, op, m, tid);
+        }
         break;
       case F_DENSE2:
       case F_SWAP2: {
@@ -348,27 +399,31 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         break;
       }
       case F_DENSE3: {
-        const uint32_t pm = (1u << op.pos[0]) | (1u << op.pos[1]) | (1u << op.pos[2]);
-        switch (pm) {
+        if constexpr (DK) {
+          const uint32_t pm = (1u << op.pos[0]) | (1u << op.pos[1]) | (1u << op.pos[2]);
+          switch (pm) {
 #define QS_D3(M) \
   case M:        \
     if constexpr ((M >> RB) == 0) do_densek<R, NR, 3, M>(a, m, op.creg, tok); \
     break;
-          QS_D3(7u) QS_D3(11u) QS_D3(13u) QS_D3(14u) QS_D3(19u) QS_D3(21u) QS_D3(22u) QS_D3(25u) QS_D3(26u)
-          QS_D3(28u)
+            QS_D3(7u) QS_D3(11u) QS_D3(13u) QS_D3(14u) QS_D3(19u) QS_D3(21u) QS_D3(22u) QS_D3(25u) QS_D3(26u)
+            QS_D3(28u)
 #undef QS_D3
+          }
         }
         break;
       }
       case F_DENSE4: {
-        const uint32_t pm = (1u << op.pos[0]) | (1u << op.pos[1]) | (1u << op.pos[2]) | (1u << op.pos[3]);
-        switch (pm) {
+        if constexpr (DK) {
+          const uint32_t pm = (1u << op.pos[0]) | (1u << op.pos[1]) | (1u << op.pos[2]) | (1u << op.pos[3]);
+          switch (pm) {
 #define QS_D4(M) \
   case M:        \
     if constexpr ((M >> RB) == 0) do_densek<R, NR, 4, M>(a, m, op.creg, tok); \
     break;
-          QS_D4(15u) QS_D4(23u) QS_D4(27u) QS_D4(29u) QS_D4(30u)
+            QS_D4(15u) QS_D4(23u) QS_D4(27u) QS_D4(29u) QS_D4(30u)
 #undef QS_D4
+          }
         }
         break;
       }
@@ -376,17 +431,104 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         break;
     }
   }
+  if (fdirty) flush_lane<R, NR>(a, f);
+}
 
-  // store in the final layout (whose thread bits include the low contiguous bits)
-  gthr = 0;
+// One pass over all 2^(n-K) tiles.  PF: persistent workgroups that prefetch
+// the next tile into a second register set while the current one is computed
+// (grid = resident workgroups); otherwise one tile per workgroup.
+template <typename R, int RB, int TB, bool DK, bool PF>
+__global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const PassHdr* __restrict__ H,
+                                                   const FOp* __restrict__ ops, const cx<R>* __restrict__ coef,
+                                                   uint64_t ntiles) {
+  constexpr int NR = 1 << RB;
+  constexpr int K = RB + TB;
+  extern __shared__ __align__(16) char lds_raw[];
+  cx<R>* lds = reinterpret_cast<cx<R>*>(lds_raw);
+  const uint32_t tid = threadIdx.x;
+  const FOp* P = ops + H->op0;
+  const cx<R>* C;
+  if constexpr (!DK) {
+    // Stage the pass coefficients in LDS once per workgroup: op payload reads are
+    // then LDS reads, which never wait on the prefetched global tile loads.
+    cx<R>* lc = lds + (H->uses_lds ? (1 << K) : 0);
+    const cx<R>* gc = coef + H->coef0;
+    const int nc = H->ncoef;
+    for (int i = (int)tid; i < nc; i += (int)blockDim.x) lc[i] = gc[i];
+    __syncthreads();
+    C = lc;
+  } else {
+    C = coef + H->coef0;
+  }
+  // per-lane parts of the load and store addresses, per-register parts as bit masks
+  uint64_t gin = 0, gout = 0;
 #pragma unroll
-  for (int b = 0; b < TB; ++b) gthr |= (uint64_t)((tid >> b) & 1u) << H->tbits[H->ts1[b]];
+  for (int b = 0; b < TB; ++b) {
+    const uint64_t bit = (tid >> b) & 1u;
+    gin |= bit << H->tbits[H->ts0[b]];
+    gout |= bit << H->tbits[H->ts1[b]];
+  }
+  uint64_t min_[RB], mout[RB];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    uint64_t g = 0;
+  for (int b = 0; b < RB; ++b) {
+    min_[b] = 1ull << H->tbits[H->rs0[b]];
+    mout[b] = 1ull << H->tbits[H->rs1[b]];
+  }
+  cx<R> a[NR];
+  [[maybe_unused]] cx<R> nx[PF ? NR : 1];
+  uint64_t t = blockIdx.x;
+  if constexpr (PF) {
+    if (t < ntiles) {
+      const uint64_t base = insert_zeros(t, H->sorted, K) | gin;
 #pragma unroll
-    for (int b = 0; b < RB; ++b) g |= (uint64_t)((r >> b) & 1) << H->tbits[H->rs1[b]];
-    st(v + (tbase | gthr | g), a[r]);
+      for (int r = 0; r < NR; ++r) {
+        uint64_t g = 0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+          if ((r >> b) & 1) g |= min_[b];
+        a[r] = ld_nt(v + (base | g));
+      }
+    }
+  }
+  for (; t < ntiles; t += gridDim.x) {
+    if constexpr (PF) {
+      const uint64_t tn = t + gridDim.x;
+      if (tn < ntiles) {
+        const uint64_t base = insert_zeros(tn, H->sorted, K) | gin;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          uint64_t g = 0;
+#pragma unroll
+          for (int b = 0; b < RB; ++b)
+            if ((r >> b) & 1) g |= min_[b];
+          nx[r] = ld_nt(v + (base | g));
+        }
+      }
+    } else {
+      const uint64_t base = insert_zeros(t, H->sorted, K) | gin;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        uint64_t g = 0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+          if ((r >> b) & 1) g |= min_[b];
+        a[r] = ld_nt(v + (base | g));
+      }
+    }
+    run_ops<R, RB, TB, DK>(a, lds, H, P, C, tid);
+    const uint64_t base = insert_zeros(t, H->sorted, K) | gout;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      uint64_t g = 0;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+        if ((r >> b) & 1) g |= mout[b];
+      st_nt(v + (base | g), a[r]);
+    }
+    if constexpr (PF) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) a[r] = nx[r];
+    }
   }
 }
 
@@ -404,11 +546,18 @@ static uint64_t gate_mask(const qs_gate& g) {
   return m;
 }
 
+// Upper bound of the coefficient entries a gate's op can need.
+static int coef_cost(const qs_gate& g, int NR) {
+  if (gate_diag_like(g)) return 4 * NR;
+  if (g.kind == QS_GATE_XPERM || g.kind == QS_GATE_SWAP) return 2;
+  return 1 << (2 * g.ntargets);
+}
+
 static bool fusable(const qs_gate& g, const Config& c) {
   const int K = c.RB + c.TB;
   if (g.ncontrols + g.ntargets > K) return false;
   if (g.kind == QS_GATE_DENSE && g.ntargets > std::min(4, c.RB)) return false;
-  if (gate_diag_like(g) && g.ntargets > std::min(4, c.RB + 2)) return false;
+  if (gate_diag_like(g) && g.ncontrols + g.ntargets > c.RB + 2) return false;
   return true;
 }
 
@@ -464,11 +613,13 @@ static std::vector<int> needed_regs(const qs_gate& g, const std::vector<int>& lo
   std::vector<int> need;
   const int c = g.ncontrols, t = g.ntargets;
   if (gate_diag_like(g)) {
+    // at most two of the gate's bits (controls or targets) may stay thread bits
     int outside = 0;
-    for (int i = 0; i < t; ++i)
-      if (index_in(cur.rs, loc[c + i]) < 0) ++outside;
+    for (int i = 0; i < c + t; ++i)
+      if (index_in(cur.rs, loc[i]) < 0) ++outside;
     if (outside <= 2) return need;
     for (int i = 0; i < t; ++i) need.push_back(loc[c + i]);
+    for (int i = 0; i < c && (int)need.size() < c + t - 2; ++i) need.push_back(loc[i]);
     return need;
   }
   for (int i = 0; i < t; ++i) need.push_back(loc[c + i]);
@@ -476,58 +627,68 @@ static std::vector<int> needed_regs(const qs_gate& g, const std::vector<int>& lo
 }
 
 template <typename R>
-static int build_op(const qs_gate& g, const std::vector<int>& loc, const Layout& cur, OpBuild& ob) {
+static int build_op(const qs_gate& g, const std::vector<int>& loc, const Layout& cur, int NR, OpBuild& ob) {
   FOp& op = ob.op;
   std::memset(&op, 0, sizeof(op));
   const int c = g.ncontrols, t = g.ntargets;
-  // controls
+  ob.coef.clear();
+  if (gate_diag_like(g)) {
+    // Diagonal gate (controls included): every gate bit that is a register bit is
+    // resolved here per register; the (<= 2) thread bits select among 2^ns
+    // values per register at run time.  Table T[sel * NR + r]; op.creg = mask
+    // of registers that are not multiplied by 1.
+    std::vector<int> selb;  // gate-bit indices living on thread bits
+    for (int i = 0; i < c + t; ++i)
+      if (index_in(cur.rs, loc[i]) < 0) selb.push_back(i);
+    if (selb.size() > 2) return QS_ERR_UNSUPPORTED;
+    const int ns = (int)selb.size();
+    op.kind = F_DIAG;
+    op.ntt = ns;
+    for (int k = 0; k < ns; ++k) op.tpos[k] = index_in(cur.ts, loc[selb[k]]);
+    const int D = 1 << t;
+    ob.coef.assign(2 * (size_t)NR * (1 << ns), 0.0);
+    uint32_t mask = 0;
+    for (int sel = 0; sel < (1 << ns); ++sel)
+      for (int r = 0; r < NR; ++r) {
+        bool ctrl_ok = true;
+        int m = 0;
+        for (int i = 0; i < c + t; ++i) {
+          int bit;
+          const int si = index_in(selb, i);
+          if (si >= 0) bit = (sel >> (ns - 1 - si)) & 1;
+          else bit = (r >> index_in(cur.rs, loc[i])) & 1;
+          if (i < c) {
+            if (!bit) ctrl_ok = false;
+          } else if (bit) {
+            m |= 1 << (t - 1 - (i - c));
+          }
+        }
+        double re = 1.0, im = 0.0;
+        if (ctrl_ok) {
+          if (g.kind == QS_GATE_PHASE) {
+            if (m == D - 1) {
+              re = g.m[0];
+              im = g.m[1];
+            }
+          } else {
+            re = g.m[2 * m];
+            im = g.m[2 * m + 1];
+          }
+        }
+        const size_t e = (size_t)sel * NR + r;
+        ob.coef[2 * e] = re;
+        ob.coef[2 * e + 1] = im;
+        if (re != 1.0 || im != 0.0) mask |= 1u << r;
+      }
+    op.creg = mask;
+    return QS_OK;
+  }
+  // controls of a non-diagonal gate: register-space mask + lane predicate
   for (int i = 0; i < c; ++i) {
     const int b = loc[i];
     const int ri = index_in(cur.rs, b);
     if (ri >= 0) op.creg |= 1u << ri;
     else op.cthr |= 1u << index_in(cur.ts, b);
-  }
-  ob.coef.clear();
-  if (g.kind == QS_GATE_PHASE || gate_diag_like(g)) {
-    // Diagonal over the target bits.  PHASE = diag(1, .., 1, v): the last target
-    // becomes a control-like selector: entries 1 except the all-ones one.
-    const int D = 1 << t;
-    std::vector<double> dg(2 * D);
-    for (int m = 0; m < D; ++m) {
-      if (g.kind == QS_GATE_PHASE) {
-        dg[2 * m] = (m == D - 1) ? g.m[0] : 1.0;
-        dg[2 * m + 1] = (m == D - 1) ? g.m[1] : 0.0;
-      } else {
-        dg[2 * m] = g.m[2 * m];
-        dg[2 * m + 1] = g.m[2 * m + 1];
-      }
-    }
-    // split targets into register and thread bits
-    std::vector<int> rt, tt;  // original target indices
-    for (int i = 0; i < t; ++i) {
-      if (index_in(cur.rs, loc[c + i]) >= 0) rt.push_back(i);
-      else tt.push_back(i);
-    }
-    if (tt.size() > 2) return QS_ERR_UNSUPPORTED;
-    op.kind = F_DIAG;
-    op.npos = (int)rt.size();
-    for (size_t k = 0; k < rt.size(); ++k) op.pos[k] = index_in(cur.rs, loc[c + rt[k]]);
-    op.ntt = (int)tt.size();
-    for (size_t k = 0; k < tt.size(); ++k) op.tpos[k] = index_in(cur.ts, loc[c + tt[k]]);
-    const int nr = (int)rt.size(), nt = (int)tt.size();
-    ob.coef.assign(2 * D, 0.0);
-    for (int ridx = 0; ridx < (1 << nr); ++ridx)
-      for (int tsel = 0; tsel < (1 << nt); ++tsel) {
-        int m = 0;
-        for (int k = 0; k < nr; ++k)
-          if ((ridx >> (nr - 1 - k)) & 1) m |= 1 << (t - 1 - rt[k]);
-        for (int k = 0; k < nt; ++k)
-          if ((tsel >> (nt - 1 - k)) & 1) m |= 1 << (t - 1 - tt[k]);
-        const int e = (ridx << nt) | tsel;
-        ob.coef[2 * e] = dg[2 * m];
-        ob.coef[2 * e + 1] = dg[2 * m + 1];
-      }
-    return QS_OK;
   }
   // non-diagonal: all targets are register bits
   std::vector<int> rp(t);
@@ -577,6 +738,166 @@ static int build_op(const qs_gate& g, const std::vector<int>& loc, const Layout&
   return QS_OK;
 }
 
+// ---- peephole merging of built ops ---------------------------------------------
+//
+// Ops act on tile-local bit sets; two ops on disjoint sets commute.  A new op is
+// merged into the most recent earlier op that shares a bit with it (everything
+// in between commutes with it), never across a relayout:
+//   diag  x diag           -> one diagonal table (thread selector union <= 2 bits)
+//   dense1 x dense1        -> product matrix (same register position and controls)
+//   dense1 x diag on that register bit (no controls) -> product matrix
+// Random circuits put two or more gates on the same qubit inside most passes,
+// so this removes a large share of the ops.
+
+struct Built {
+  FOp op;
+  std::vector<double> coef;  // (re, im) pairs
+  uint32_t bits = 0;         // tile-local bits the op depends on / changes
+  std::vector<int> selb;     // DIAG: tile-local thread bits selecting the value, MSB first
+  Layout lay;                // layout in effect when the op runs
+};
+
+using cplx = std::pair<double, double>;
+static inline cplx cm_(cplx a, cplx b) { return {a.first * b.first - a.second * b.second, a.first * b.second + a.second * b.first}; }
+
+static void mat2_mul(const double* A, const double* B, double* out) {  // out = A * B (2x2 complex)
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 2; ++k) {
+        const double ar = A[2 * (i * 2 + k)], ai = A[2 * (i * 2 + k) + 1];
+        const double br = B[2 * (k * 2 + j)], bi = B[2 * (k * 2 + j) + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      out[2 * (i * 2 + j)] = re;
+      out[2 * (i * 2 + j) + 1] = im;
+    }
+}
+
+// DIAG op confined to one register bit: returns the register position and (d0, d1), else -1.
+static int diag_single_reg(const Built& b, int NR, double* d2) {
+  if (b.op.kind != F_DIAG || !b.selb.empty() || __builtin_popcount(b.bits) != 1) return -1;
+  const int tb = __builtin_ctz(b.bits);
+  const int p = index_in(b.lay.rs, tb);
+  if (p < 0) return -1;
+  for (int r = 0; r < NR; ++r) {
+    const int r0 = r & (1 << p);
+    if (b.coef[2 * r] != b.coef[2 * r0] || b.coef[2 * r + 1] != b.coef[2 * r0 + 1]) return -1;
+  }
+  d2[0] = b.coef[0];
+  d2[1] = b.coef[1];
+  d2[2] = b.coef[2 * (1 << p)];
+  d2[3] = b.coef[2 * (1 << p) + 1];
+  return p;
+}
+
+static void diag_to_dense1(const double* d2, double* M) {
+  M[0] = d2[0]; M[1] = d2[1]; M[2] = 0; M[3] = 0;
+  M[4] = 0; M[5] = 0; M[6] = d2[2]; M[7] = d2[3];
+}
+
+static bool merge_diag(Built& e, const Built& d, int NR) {
+  std::vector<int> u = e.selb;
+  for (int b : d.selb)
+    if (index_in(u, b) < 0) u.push_back(b);
+  if (u.size() > 2) return false;
+  const int nu = (int)u.size();
+  auto proj = [&](const std::vector<int>& sb, int usel) {
+    int s = 0;
+    for (size_t k = 0; k < sb.size(); ++k) {
+      const int ui = index_in(u, sb[k]);
+      s = (s << 1) | ((usel >> (nu - 1 - ui)) & 1);
+    }
+    return s;
+  };
+  std::vector<double> T(2 * (size_t)NR * (1 << nu));
+  for (int us = 0; us < (1 << nu); ++us)
+    for (int r = 0; r < NR; ++r) {
+      const size_t ie = (size_t)proj(e.selb, us) * NR + r, id = (size_t)proj(d.selb, us) * NR + r;
+      const cplx v = cm_({e.coef[2 * ie], e.coef[2 * ie + 1]}, {d.coef[2 * id], d.coef[2 * id + 1]});
+      T[2 * ((size_t)us * NR + r)] = v.first;
+      T[2 * ((size_t)us * NR + r) + 1] = v.second;
+    }
+  e.coef.swap(T);
+  e.selb = u;
+  e.bits |= d.bits;
+  return true;
+}
+
+static bool try_merge(std::vector<Built>& out, Built& d, const Layout& cur, int NR) {
+  (void)cur;
+  if (d.op.kind != F_DIAG && d.op.kind != F_DENSE1) return false;
+  int k = (int)out.size() - 1;
+  for (; k >= 0; --k) {
+    if (out[k].op.kind == F_RELAYOUT) return false;
+    if (out[k].bits & d.bits) break;
+  }
+  if (k < 0) return false;
+  Built& e = out[k];
+  if (d.op.kind == F_DIAG && e.op.kind == F_DIAG) return merge_diag(e, d, NR);
+  double d2[4], M[8];
+  if (d.op.kind == F_DENSE1 && e.op.kind == F_DENSE1 && e.op.pos[0] == d.op.pos[0] && e.op.creg == d.op.creg &&
+      e.op.cthr == d.op.cthr) {
+    mat2_mul(d.coef.data(), e.coef.data(), M);
+    e.coef.assign(M, M + 8);
+    return true;
+  }
+  if (d.op.kind == F_DENSE1 && d.op.creg == 0 && d.op.cthr == 0) {
+    const int p = diag_single_reg(e, NR, d2);
+    if (p == d.op.pos[0]) {
+      double D[8];
+      diag_to_dense1(d2, D);
+      mat2_mul(d.coef.data(), D, M);
+      Built nb = d;
+      nb.coef.assign(M, M + 8);
+      out[k] = std::move(nb);
+      return true;
+    }
+    return false;
+  }
+  if (d.op.kind == F_DIAG && e.op.kind == F_DENSE1 && e.op.creg == 0 && e.op.cthr == 0) {
+    const int p = diag_single_reg(d, NR, d2);
+    if (p == e.op.pos[0]) {
+      double D[8];
+      diag_to_dense1(d2, D);
+      mat2_mul(D, e.coef.data(), M);
+      e.coef.assign(M, M + 8);
+      return true;
+    }
+  }
+  return false;
+}
+
+// Final encoding of a DIAG op: tpos from the op's layout; register-independent
+// tables collapse to one value per selector (op.pure = 1, lane mode).
+static void finalize_diag(Built& b, int NR) {
+  const int ns = (int)b.selb.size();
+  b.op.ntt = ns;
+  for (int k = 0; k < ns; ++k) b.op.tpos[k] = index_in(b.lay.ts, b.selb[k]);
+  bool lane = true;
+  uint32_t mask = 0;
+  for (int s = 0; s < (1 << ns); ++s)
+    for (int r = 0; r < NR; ++r) {
+      const size_t i = (size_t)s * NR + r, i0 = (size_t)s * NR;
+      if (b.coef[2 * i] != b.coef[2 * i0] || b.coef[2 * i + 1] != b.coef[2 * i0 + 1]) lane = false;
+      if (b.coef[2 * i] != 1.0 || b.coef[2 * i + 1] != 0.0) mask |= 1u << r;
+    }
+  if (lane) {
+    std::vector<double> T(2 * (1 << ns));
+    for (int s = 0; s < (1 << ns); ++s) {
+      T[2 * s] = b.coef[2 * (size_t)s * NR];
+      T[2 * s + 1] = b.coef[2 * (size_t)s * NR + 1];
+    }
+    b.coef.swap(T);
+    b.op.pure = 1;
+    b.op.creg = mask ? (uint32_t)((1ull << NR) - 1) : 0u;
+  } else {
+    b.op.pure = 0;
+    b.op.creg = mask;
+  }
+}
+
 struct PassPlan {
   bool fused;
   int single;                      // gate index when !fused
@@ -602,26 +923,23 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
       i = 1;
     } else {
       uint64_t S = low, blocked = 0;
+      int coef_budget = FMAX_PASS_COEF;  // LDS-staged payload per pass (complex entries)
       for (; i < remaining.size(); ++i) {
         const int gi = remaining[i];
         const qs_gate& g = gates[gi];
         const uint64_t gm = gate_mask(g);
-        if ((gm & blocked) || !fusable(g, c) || __builtin_popcountll(S | gm) > K) {
+        const int cost = coef_cost(g, 1 << c.RB);
+        if ((gm & blocked) || !fusable(g, c) || __builtin_popcountll(S | gm) > K || cost > coef_budget) {
           blocked |= gm;  // everything sharing a bit with a deferred gate must stay behind it
           rest.push_back(gi);
           continue;
         }
         S |= gm;
+        coef_budget -= cost;
         p.gates.push_back(gi);
       }
-      // fill the tile to K bits: prefer bits the deferred gates use first
-      for (size_t j = 0; j < rest.size() && __builtin_popcountll(S) < K; ++j) {
-        uint64_t gm = gate_mask(gates[rest[j]]);
-        while (gm && __builtin_popcountll(S) < K) {
-          S |= 1ull << __builtin_ctzll(gm);
-          gm &= gm - 1;
-        }
-      }
+      // fill the tile to K bits with the lowest unused bits: that lengthens the
+      // contiguous runs each workgroup reads and writes (DRAM page locality)
       for (int b = 0; b < n && __builtin_popcountll(S) < K; ++b) S |= 1ull << b;
       for (int b = 0; b < n; ++b)
         if ((S >> b) & 1) p.tbits.push_back(b);  // ascending; the first L are the low bits
@@ -692,6 +1010,8 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   for (int b = 0; b < c.TB; ++b) H.ts0[b] = cur.ts[b];
   const int op_start = (int)ops.size();
   const int coef_start = (int)(coef.size() / 2);
+  const int NR = 1 << c.RB;
+  std::vector<Built> built;
   for (size_t k = 0; k < p.gates.size(); ++k) {
     const qs_gate& g = gates[p.gates[k]];
     std::vector<int> need = needed_regs(g, loc[k], cur);
@@ -702,23 +1022,40 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
       std::vector<int> la = lookahead_from(k + 1);
       std::vector<int> nr = choose_regs(need, la, cur.rs, c.RB, K, c.L);
       cur = make_layout(nr, K);
-      FOp rl;
-      std::memset(&rl, 0, sizeof(rl));
-      rl.kind = F_RELAYOUT;
-      for (int b = 0; b < c.RB; ++b) rl.rs[b] = cur.rs[b];
-      for (int b = 0; b < c.TB; ++b) rl.ts[b] = cur.ts[b];
-      ops.push_back(rl);
+      Built rl;
+      std::memset(&rl.op, 0, sizeof(rl.op));
+      rl.op.kind = F_RELAYOUT;
+      for (int b = 0; b < c.RB; ++b) rl.op.rs[b] = cur.rs[b];
+      for (int b = 0; b < c.TB; ++b) rl.op.ts[b] = cur.ts[b];
+      built.push_back(rl);
       H.uses_lds = 1;
     }
     OpBuild ob;
-    int rc = build_op<R>(g, loc[k], cur, ob);
+    int rc = build_op<R>(g, loc[k], cur, NR, ob);
     if (rc) {
       set_error("fusion planner produced an op the tile kernel cannot encode");
       return rc;
     }
-    ob.op.coef = (int)(coef.size() / 2) - coef_start;
-    coef.insert(coef.end(), ob.coef.begin(), ob.coef.end());
-    ops.push_back(ob.op);
+    Built b;
+    b.op = ob.op;
+    b.coef = std::move(ob.coef);
+    b.lay = cur;
+    b.bits = 0;
+    for (int i = 0; i < g.ncontrols + g.ntargets; ++i) b.bits |= 1u << loc[k][i];
+    if (b.op.kind == F_DIAG)
+      for (int i = 0; i < g.ncontrols + g.ntargets; ++i)
+        if (index_in(cur.rs, loc[k][i]) < 0) b.selb.push_back(loc[k][i]);
+    if (!try_merge(built, b, cur, NR)) built.push_back(std::move(b));
+  }
+  for (Built& b : built) {
+    if (b.op.kind == F_DIAG) {
+      finalize_diag(b, NR);
+      if (b.op.creg == 0) continue;  // the merged product is the identity
+    }
+    if (b.op.kind == F_DENSE3 || b.op.kind == F_DENSE4) H.has_dk = 1;
+    b.op.coef = (int)(coef.size() / 2) - coef_start;
+    coef.insert(coef.end(), b.coef.begin(), b.coef.end());
+    ops.push_back(b.op);
   }
   // final layout must keep the low L bits on threads
   bool low_in_regs = false;
@@ -743,6 +1080,7 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   H.op0 = op_start;
   H.nops = (int)ops.size() - op_start;
   H.coef0 = coef_start;
+  H.ncoef = (int)(coef.size() / 2) - coef_start;
   return QS_OK;
 }
 
@@ -816,10 +1154,22 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
       continue;
     }
     const int h = hdr_of[i];
-    const size_t lds = hdrs[h].uses_lds ? lds_bytes : 0;
+    const size_t lds_relayout = hdrs[h].uses_lds ? lds_bytes : 0;
+    const size_t lds = hdrs[h].has_dk ? lds_relayout : lds_relayout + (size_t)hdrs[h].ncoef * sizeof(cx<R>);
     rc = prof_begin(ctx, PROF_FUSED, s);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_fused<R, RB, TB>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h, dO, dC);
+    if (hdrs[h].has_dk) {
+      hipLaunchKernelGGL((k_fused<R, RB, TB, true, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h,
+                         dO, dC, tiles);
+    } else if (ctx->fused_prefetch) {
+      // persistent, prefetching: resident workgroups only
+      const uint64_t grid = std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * ctx->fused_wg_per_cu);
+      hipLaunchKernelGGL((k_fused<R, RB, TB, false, true>), dim3((unsigned)grid), dim3(1 << TB), lds, s, v, dH + h,
+                         dO, dC, tiles);
+    } else {
+      hipLaunchKernelGGL((k_fused<R, RB, TB, false, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h,
+                         dO, dC, tiles);
+    }
     QS_CHECK_HIP(hipGetLastError());
     rc = prof_end(ctx, s);
     if (rc) return rc;
@@ -834,8 +1184,13 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
 template <typename R, int RB, int TB> static void set_lds_attr() {
   static bool done = false;
   if (done) return;
-  (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)((1u << (RB + TB)) * sizeof(cx<R>)));
+  const int bytes = (int)((1u << (RB + TB)) * sizeof(cx<R>) + FMAX_PASS_COEF * sizeof(cx<R>));
+  (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB, true, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB, false, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB, false, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   done = true;
 }
 

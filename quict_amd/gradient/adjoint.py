@@ -244,18 +244,32 @@ def adjoint_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, p
 
 def parameter_shift_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, precision: str = "double",
                              device: int = 0, params: Optional[Sequence[int]] = None) -> np.ndarray:
+    """Two-term rule for exp(-i t P/2) gates (P a Pauli string); the four-term
+    rule c+ [E(t+pi/2) - E(t-pi/2)] - c- [E(t+3pi/2) - E(t-3pi/2)],
+    c+- = (sqrt2 +- 1)/(4 sqrt2), for controlled rotations (CRz), whose generator
+    has eigenvalues {0, 0, +-1/2}."""
     theta = np.asarray(theta, dtype=np.float64)
+    types = {}
     for gtype, _, p, _ in pc.ops:
-        if p is not None and gtype not in SHIFT_RULE_TYPES:
-            raise qerr.ValueError(f"the two-term shift rule does not hold for {gtype.name}")
+        if p is not None:
+            if gtype not in SHIFT_RULE_TYPES:
+                raise qerr.ValueError(f"no shift rule for {gtype.name}")
+            types[p] = gtype
     idx = range(pc.nparams) if params is None else params
     g = np.zeros(pc.nparams)
+
+    def shifted(k, s):
+        t = theta.copy()
+        t[k] += s
+        return expectation(pc, t, H, precision, device)
+
+    cp, cm = (np.sqrt(2) + 1) / (4 * np.sqrt(2)), (np.sqrt(2) - 1) / (4 * np.sqrt(2))
     for k in idx:
-        tp = theta.copy()
-        tp[k] += np.pi / 2
-        tm = theta.copy()
-        tm[k] -= np.pi / 2
-        g[k] = 0.5 * (expectation(pc, tp, H, precision, device) - expectation(pc, tm, H, precision, device))
+        if types.get(k) == GateType.crz:
+            g[k] = cp * (shifted(k, np.pi / 2) - shifted(k, -np.pi / 2)) - \
+                cm * (shifted(k, 3 * np.pi / 2) - shifted(k, -3 * np.pi / 2))
+        else:
+            g[k] = 0.5 * (shifted(k, np.pi / 2) - shifted(k, -np.pi / 2))
     return g
 
 

@@ -32,3 +32,14 @@ def test_shift_rule_vs_finite_differences_hea():
     ps = og.parameter_shift(pc, th, terms)
     fd = og.finite_difference(pc, th, terms)
     assert np.max(np.abs(ps - fd)) < 1e-7
+
+
+def test_four_term_rule_for_controlled_rotation():
+    # CRz generator has eigenvalues {0, 0, +-1/2}: the two-term rule is wrong, the four-term rule exact
+    pc = ParameterizedCircuit(2).add(GateType.h, [0]).add(GateType.ry, [1], value=0.7).add(GateType.crz, [0, 1], param=0)
+    pc.add(GateType.h, [1])
+    terms = [(1.0, "XZ"), (0.3, "IX")]
+    for th in [0.4, 2.9]:
+        ps = og.parameter_shift(pc, [th], terms)[0]
+        fd = og.finite_difference(pc, [th], terms)[0]
+        assert abs(ps - fd) < 1e-8
