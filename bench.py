@@ -63,6 +63,156 @@ def run_sv(n, circuit, precision, fusion, steps, warmup, lib, ctx):
     return dt, passes, ms, cnt
 
 
+def run_stream_copy(n, lib, ctx, reps=5):
+    """Plain streaming copy of one state (2^n complex128) with the library's copy kernel:
+    the achievable-bandwidth reference for the tile passes."""
+    import torch
+    from quict_amd import _native as nat
+    src = torch.zeros(1 << n, dtype=torch.complex128, device="cuda")
+    dst = torch.empty_like(src)
+    nbytes = (1 << n) * 16
+    stream = nat.stream_ptr(0)
+    out = {}
+    for nt in (0, 1):
+        for _ in range(2):
+            nat.check(lib.qs_stream_copy(ctx, ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                                         nbytes, nt, stream), "qs_stream_copy")
+        torch.cuda.synchronize()
+        profile_read(lib, ctx, reset=True)
+        lib.qs_profile_enable(ctx, 1)
+        for _ in range(reps):
+            nat.check(lib.qs_stream_copy(ctx, ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                                         nbytes, nt, stream), "qs_stream_copy")
+        torch.cuda.synchronize()
+        lib.qs_profile_enable(ctx, 0)
+        ms, cnt = profile_read(lib, ctx, reset=True)
+        avg = ms[1] / max(cnt[1], 1)
+        out["nontemporal" if nt else "plain"] = {"avg_ms": avg, "gbs": 2 * nbytes / (avg * 1e-3) / 1e9}
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_adjoint_20q(reps=3):
+    """20-qubit 12-layer hardware-efficient ansatz, 480 parameters, H = sum ZZ + 0.5 sum X:
+    adjoint gradient vs the parameter-shift gradient, both on this GPU."""
+    import numpy as np
+    from quict_amd import workloads
+    from quict_amd.gradient.adjoint import (ParameterizedCircuit, PauliSum, adjoint_gradient,
+                                            parameter_shift_gradient)
+    n, layers = 20, 12
+    pc = ParameterizedCircuit.hea(n, layers)
+    th = workloads.hea_params(n, layers, 3).ravel()
+    H = PauliSum(n, workloads.zz_x_hamiltonian(n))
+    adjoint_gradient(pc, th, H)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        E, g = adjoint_gradient(pc, th, H)
+    t_adj = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    ps = parameter_shift_gradient(pc, th, H)
+    t_ps = time.perf_counter() - t0
+    err = float(np.max(np.abs(g - ps)))
+    return {"qubits": n, "params": pc.nparams, "energy": E, "adjoint_ms": t_adj * 1e3,
+            "adjoint_grads_per_s": pc.nparams / t_adj, "shift_ms": t_ps * 1e3,
+            "shift_grads_per_s": pc.nparams / t_ps, "speedup_vs_shift": t_ps / t_adj,
+            "max_abs_err_vs_shift": err}
+
+
+def run_qnn_batch(precision, reps=3, sample_circuits=3, sample_params=6):
+    """1024 circuits, widths 8..18, 16 Ry-encoded features, 6 layers, loss mean((<Z0>-t)^2):
+    one autotuned persistent launch for forward + backward, against per-circuit parameter
+    shift on this GPU (a sample of circuits and parameters, scaled to a rate)."""
+    import numpy as np
+    from quict_amd.gradient.qnn import QNNBatch, synthetic_batch, circuit_for
+    from quict_amd.gradient.adjoint import PauliSum, parameter_shift_gradient
+    widths, x, t = synthetic_batch(1024, 16, 8, 18, 4)
+    layers, mw = 6, 18
+    theta = np.random.RandomState(7).uniform(-np.pi, np.pi, layers * 2 * mw)
+    qb = QNNBatch(widths, x, t, layers=layers, max_width=mw, precision=precision)
+    t0 = time.perf_counter()
+    cfg = qb.autotune(theta, reps=2, use_cache=False)
+    t_tune = time.perf_counter() - t0
+    qb.forward_backward(theta)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        qb.forward_backward(theta)
+    t_b = (time.perf_counter() - t0) / reps
+    derivs = int(sum(2 * layers * int(w) for w in widths))  # ansatz derivatives over the batch
+    # per-circuit parameter shift on the same GPU, and the batched gradient of one circuit
+    err = 0.0
+    t_ps = 0.0
+    n_ps = 0
+    for c in range(sample_circuits):
+        w = int(widths[c])
+        pc = circuit_for(w, x[c], layers, mw)
+        params = [(l * 2 + k) * mw + q for l in range(layers) for k in range(2) for q in range(w)][:sample_params]
+        t0 = time.perf_counter()
+        ps = parameter_shift_gradient(pc, theta, PauliSum(w, [(1.0, "Z" + "I" * (w - 1))]), precision,
+                                      params=params)
+        t_ps += time.perf_counter() - t0
+        n_ps += len(params)
+        one = QNNBatch(widths[c:c + 1], x[c:c + 1], t[c:c + 1], layers=layers, max_width=mw, precision=precision)
+        _, ev1, g1, _ = one.forward_backward(theta)
+        scale = 2 * (ev1[0] - t[c])
+        err = max(err, float(np.max(np.abs(g1[params] - scale * ps[params]))))
+    shift_rate = n_ps / t_ps
+    return {"circuits": len(widths), "config": cfg, "autotune_s": t_tune, "ms_per_batch": t_b * 1e3,
+            "circuits_per_s": len(widths) / t_b, "grads_per_s": derivs / t_b,
+            "shift_grads_per_s": shift_rate, "speedup_vs_shift": (derivs / t_b) / shift_rate,
+            "max_abs_err_vs_shift": err,
+            "note": f"shift rate from {sample_circuits} circuits x {sample_params} parameters, two GPU "
+                    "expectations each; error compares the batched gradient of one circuit with "
+                    "2(E - t) * shift derivative"}
+
+
+def run_dm14(lib, ctx, reps=3):
+    """14-qubit noisy density matrix (2^28 complex128 entries): 300-gate circuit with
+    depolarizing / amplitude-damping channels, then diagonal and trace."""
+    import torch
+    from quict_amd import workloads
+    from quict_amd.simulation import DensityMatrixSimulator
+    c = workloads.dm14_circuit()
+    nm = workloads.dm_noise_model()
+    sim = DensityMatrixSimulator("GPU", precision="double", accumulated_mode=True)
+    sim.run(c, noise_model=nm)
+    torch.cuda.synchronize()
+    profile_read(lib, ctx, reset=True)
+    lib.qs_profile_enable(ctx, 1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.run(c, noise_model=nm)
+        d = sim.diagonal()
+        tr = sim.trace()
+    dt = (time.perf_counter() - t0) / reps
+    lib.qs_profile_enable(ctx, 0)
+    ms, cnt = profile_read(lib, ctx, reset=True)
+    pass_bytes = 2 * (1 << 28) * 16
+    passes = sim.last_passes
+    kern_ms = (ms[0] + ms[1]) / reps
+    out = {"qubits": 14, "gates": 300, "passes": passes, "ms_per_run": dt * 1e3, "kernel_ms_per_run": kern_ms,
+           "gbs_wall": passes * pass_bytes / dt / 1e9, "gbs_kernels": passes * pass_bytes / (kern_ms * 1e-3) / 1e9,
+           "trace": [tr.real, tr.imag], "diag_sum": float(d.sum())}
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
+def measured_traffic(n, ngates):
+    """HBM bytes per tile-pass launch from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) when this run is the workload it was measured on."""
+    path = os.path.join(ROOT, "profiles", "k_fused_sv30_c128_traffic.txt")
+    if (n, ngates) != (30, 2000) or not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if "=" in line and not line.startswith("#"):
+                k, v = line.strip().split("=", 1)
+                vals[k] = int(v)
+    return vals.get("traffic_bytes")
+
+
 def cpu_baseline_sample(n_cpu=28, ngates=10, threads=1):
     import numpy as np
     from quict_amd import workloads
@@ -116,6 +266,8 @@ def main():
     log(f"c128 fused: {step_ms:.1f} ms/step, {passes} passes, {value:.0f} GB/s, avg tile-pass {avg_fused_ms:.3f} ms")
 
     secondary = {}
+    log("streaming copy of one state (bandwidth reference; its profile calibrates the counters)")
+    secondary["stream_copy"] = run_stream_copy(n, lib, ctx)
     if not args.no_secondary:
         log("secondary: complex64 fused")
         dt64, p64, ms64, cnt64 = run_sv(n, circuit, "single", True, 1, 1, lib, ctx)
@@ -130,10 +282,27 @@ def main():
             "gates_per_s": args.gates / dtu,
             "gate_pass_gbs": pass_bytes / (msu[1] / max(cntu[1], 1) * 1e-3) / 1e9 if cntu[1] else None,
             "note": "gate passes touch only the amplitudes a gate changes (controls / phases touch 1/2 or 1/4)"}
-        log("secondary: 20-qubit 400-gate circuit")
+        log("secondary: 20-qubit 400-gate circuit + 1024 shots")
         c20 = workloads.sv20_circuit()
         dt20, p20, _, _ = run_sv(20, c20, "double", True, 5, 2, lib, ctx)
-        secondary["sv20_400"] = {"ms_per_step": dt20 / 5 * 1e3, "gates_per_s": 400 / (dt20 / 5), "passes": p20}
+        from quict_amd.simulation import StateVectorSimulator
+        import random
+        s20 = StateVectorSimulator("GPU", precision="double")
+        s20.run(c20)
+        random.seed(1001)
+        t0 = time.perf_counter()
+        s20.sample(1024)
+        t_shots = time.perf_counter() - t0
+        del s20
+        secondary["sv20_400"] = {"ms_per_step": dt20 / 5 * 1e3, "gates_per_s": 400 / (dt20 / 5), "passes": p20,
+                                 "sample_1024_ms": t_shots * 1e3}
+        log("secondary: adjoint gradient 20q x 12 layers vs parameter shift")
+        secondary["adjoint_20q"] = run_adjoint_20q()
+        for prec in ("double", "single"):
+            log(f"secondary: batched QNN {prec} (autotune + timed)")
+            secondary[f"qnn_batch_{'c128' if prec == 'double' else 'c64'}"] = run_qnn_batch(prec)
+        log("secondary: noisy density matrix 14q")
+        secondary["dm14_noisy"] = run_dm14(lib, ctx)
 
     cpu = None
     if not args.no_cpu:
@@ -165,10 +334,13 @@ def main():
         "gates_per_s": gates_per_s,
         "passes_per_step": passes,
         "roofline": {"bound": "hbm", "achieved": roof_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (roof_achieved / HBM_PEAK_GBS) if roof_achieved else None, "traffic": None,
+                     "frac": (roof_achieved / HBM_PEAK_GBS) if roof_achieved else None,
+                     "traffic": measured_traffic(n, args.gates),
                      "kernel": "k_fused<double,4,8>", "launches_per_step": fused_launches,
                      "avg_launch_ms": avg_fused_ms,
-                     "bytes_per_launch": pass_bytes},
+                     "bytes_per_launch": pass_bytes,
+                     "frac_of_stream_copy": (roof_achieved / secondary["stream_copy"]["nontemporal"]["gbs"])
+                     if roof_achieved and "stream_copy" in secondary else None},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }

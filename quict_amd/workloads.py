@@ -38,6 +38,23 @@ def sv30_circuit(n: int = 30, ngates: int = 2000) -> Circuit:
     return random_mix_circuit(n, ngates, 2, RANDOM_GATE_PROBS_60_40)
 
 
+def dm_noise_model():
+    """Depolarizing(p=1e-3) after every one-qubit gate of the mix, amplitude
+    damping (gamma=1e-2) tensored with itself after every two-qubit gate."""
+    from .core.noise import DepolarizingError, DampingError, NoiseModel
+    names = [t.name for t in RANDOM_GATE_MIX]
+    nm = NoiseModel(basic_gates=names)
+    nm.add_noise_for_all_qubits(DepolarizingError(1e-3), names[:10])
+    d = DampingError(0.01, 0, 0)
+    nm.add_noise_for_all_qubits(d.tensor(d), names[10:])
+    return nm
+
+
+def dm14_circuit(n: int = 14, ngates: int = 300) -> Circuit:
+    """14 qubits, 300 uniform gates from the mix, seed 5 (noisy density-matrix workload)."""
+    return random_mix_circuit(n, ngates, 5)
+
+
 def hea_params(n: int, layers: int, seed: int) -> np.ndarray:
     rng = np.random.RandomState(seed)
     return rng.uniform(-np.pi, np.pi, size=(layers, 2, n))

@@ -8,7 +8,6 @@ import pytest
 
 from quict_amd.core import Circuit, H, CX, Measure
 from quict_amd.core import qasm
-from quict_amd.core.noise import DepolarizingError, DampingError, NoiseModel
 from quict_amd import workloads
 from oracle import density_matrix as odm
 from oracle import statevector as osv
@@ -26,12 +25,7 @@ def dm_sim(precision="double", fusion=True, accumulated_mode=True):
 
 
 def noise_model():
-    names = [t.name for t in workloads.RANDOM_GATE_MIX]
-    nm = NoiseModel(basic_gates=names)
-    nm.add_noise_for_all_qubits(DepolarizingError(1e-3), names[:10])
-    d = DampingError(0.01, 0, 0)
-    nm.add_noise_for_all_qubits(d.tensor(d), names[10:])
-    return nm
+    return workloads.dm_noise_model()
 
 
 @pytest.mark.parametrize("fusion", [False, True])
