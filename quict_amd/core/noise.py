@@ -41,6 +41,9 @@ def is_kraus_ops(kraus: List[np.ndarray]) -> bool:
 
 
 class QuantumNoiseError:
+    # single-qubit channels whose tensor product this channel is (set by ``tensor``)
+    _factors = None
+
     def __init__(self, ops: List[Tuple[np.ndarray, float]]):
         if not isinstance(ops, list):
             raise qerr.TypeError("QuantumNoiseError expects list[(matrix, prob)]")
@@ -101,7 +104,14 @@ class QuantumNoiseError:
         return QuantumNoiseError([(a @ b, p * q) for a, p in left for b, q in right])
 
     def tensor(self, other: "QuantumNoiseError") -> "QuantumNoiseError":
-        return QuantumNoiseError([(np.kron(a, b), p * q) for a, p in self._operators for b, q in other.operators])
+        out = QuantumNoiseError([(np.kron(a, b), p * q) for a, p in self._operators for b, q in other.operators])
+        out._factors = (self._factors or [self]) + (other._factors or [other])
+        return out
+
+    @property
+    def factors(self) -> List["QuantumNoiseError"]:
+        """The channels this one is the tensor product of (first = first qubit), or [self]."""
+        return list(self._factors) if self._factors else [self]
 
     def apply_to_gate(self, matrix: np.ndarray) -> List[np.ndarray]:
         if matrix.shape != (1 << self._qubits, 1 << self._qubits):
@@ -290,6 +300,7 @@ class NoiseModel:
                         if accumulated_mode or noise.type == NoiseChannel.damping:
                             ng = NoiseGate(noise.apply_to_gate(gate.matrix.astype(np.complex128)), len(args))
                             ng.targs = list(args)
+                            ng.source = (gate, noise)
                             out.append(ng)
                             keep = False
                         else:

@@ -58,6 +58,10 @@ class NoiseGate(Operator):
         super().__init__(args_num)
         self._noise_matrix = [np.asarray(m) for m in noise_matrix]
         self._precision = self._noise_matrix[0].dtype
+        # (gate, QuantumNoiseError) this channel was built from, when known: the
+        # density-matrix simulator then applies the gate and the channel's
+        # single-qubit factors separately (the same map, fewer operations)
+        self.source = None
 
     @property
     def noise_matrix(self) -> List[np.ndarray]:

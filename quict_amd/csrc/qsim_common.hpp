@@ -56,7 +56,7 @@ template <typename R> __device__ __forceinline__ void st(cx<R>* p, cx<R> a) {
 }
 
 // Insert a zero bit at each position of `sorted` (ascending) into g.
-__device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const int* sorted, int k) {
+__host__ __device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const int* sorted, int k) {
   for (int j = 0; j < k; ++j) {
     const int b = sorted[j];
     g = ((g >> b) << (b + 1)) | (g & ((1ull << b) - 1));

@@ -1,0 +1,819 @@
+// Host-side planner of the fused tile passes (shared structs, layout
+// bookkeeping, op construction and merging, pass grouping).  Included by
+// qsim_fused.hip, which holds the device kernels and the launch code.
+#pragma once
+#include "qsim_common.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace qs {
+
+
+enum FKind : int32_t {
+  F_RELAYOUT = 0,
+  F_DENSE1 = 1,
+  F_PERM1 = 2,   // a0' = m0*a1, a1' = m1*a0 (pure exchange when pure != 0)
+  F_DIAG = 3,
+  F_DENSE2 = 4,
+  F_SWAP2 = 5,   // a01' = m0*a10, a10' = m1*a01 (pure exchange when pure != 0)
+  F_DENSE3 = 6,
+  F_DENSE4 = 7,
+};
+
+constexpr int FMAX_TB = 10;
+constexpr int FMAX_RB = 5;
+// Complex coefficients one pass may carry (staged in LDS next to the relayout buffer).
+constexpr int FMAX_PASS_COEF = 1024;
+
+// One op of a pass program, as the host planner builds it.
+struct FOp {
+  int kind;
+  int pure;         // PERM1/SWAP2 without phases; DIAG: lane mode (one value per lane)
+  int npos;         // register positions used (targets, MSB first)
+  int ntt;          // DIAG: number of selecting thread bits (<= 2)
+  int pos[4];
+  int tpos[2];      // DIAG: selecting thread-index bits, MSB first
+  int32_t coef;     // complex offset of the payload in the pass coefficients; RELAYOUT: index into aux
+  uint32_t creg;    // control mask in register-index space ...
+  uint32_t cregv;   // ... and the values those register bits must have
+  uint32_t cthr;    // control mask in thread-index space ...
+  uint32_t cthrv;   // ... and the values those thread bits must have
+};
+
+// Its device encoding: eight 32-bit words, read with one scalar load (every
+// field a whole dword, so no sub-dword vector loads are needed):
+//   w0 = kind | pure << 8 | npos << 16 | ntt << 24
+//   w1 = pos[0] | pos[1] << 8 | pos[2] << 16 | pos[3] << 24
+//   w2 = tpos[0] | tpos[1] << 8,  w3 = coef,  w4..w7 = creg, cregv, cthr, cthrv
+struct FOpW {
+  uint32_t w[8];
+};
+static_assert(sizeof(FOpW) == 32, "FOpW layout");
+
+static FOpW encode_op(const FOp& o) {
+  FOpW e;
+  e.w[0] = (uint32_t)o.kind | (uint32_t)o.pure << 8 | (uint32_t)o.npos << 16 | (uint32_t)o.ntt << 24;
+  e.w[1] = (uint32_t)o.pos[0] | (uint32_t)o.pos[1] << 8 | (uint32_t)o.pos[2] << 16 | (uint32_t)o.pos[3] << 24;
+  e.w[2] = (uint32_t)o.tpos[0] | (uint32_t)o.tpos[1] << 8;
+  e.w[3] = (uint32_t)o.coef;
+  e.w[4] = o.creg;
+  e.w[5] = o.cregv;
+  e.w[6] = o.cthr;
+  e.w[7] = o.cthrv;
+  return e;
+}
+
+// RELAYOUT payload in the aux table (uint32): the RB register-slot and TB
+// thread-slot LDS offsets of the current layout, the flip term of the current
+// layout, then the RB + TB slot offsets of the new layout.  An offset is the
+// byte offset of the XOR-swizzled LDS position of the single tile bit the slot
+// carries; the swizzle is XOR-linear, so an amplitude's LDS address is the XOR
+// of its slots' offsets.  The write side is carried per relayout because
+// relabelings change the layout between relayouts.
+struct PassHdr {
+  int32_t nops;
+  int32_t op0;
+  int32_t coef0;
+  int32_t uses_lds;
+  int32_t has_dk;        // pass contains 3- or 4-target dense ops (kernel variant without LDS coefficients)
+  int32_t ncoef;         // complex coefficients of this pass (staged in LDS)
+  int32_t sync_store;    // store layout differs from the load layout and no relayout ordered the
+                         // loads: workgroup barrier before the stores (another wave may still have
+                         // to load an address this wave stores to)
+  int32_t tbits[16];     // tile-local bit -> global bit
+  int32_t sorted[16];    // tile bits ascending (global), for the tile base
+  int32_t rs0[FMAX_RB], ts0[FMAX_TB];  // load layout (tile-local bits)
+  int32_t rs1[FMAX_RB], ts1[FMAX_TB];  // store layout
+  uint64_t sflip;        // global-index XOR applied to every store address (pending flips)
+};
+
+// XOR fold of the 3-bit chunks above bit 3: spreads power-of-two strides over
+// LDS banks.  XOR-linear: swz(a ^ b) = swz(a) ^ swz(b).
+__host__ __device__ __forceinline__ uint32_t lds_swz(uint32_t j) {
+  uint32_t h = j >> 3, f = 0;
+  for (int i = 0; i < 5; ++i) {
+    f ^= h & 7;
+    h >>= 3;
+  }
+  return j ^ f;
+}
+
+// ---- host planner --------------------------------------------------------------
+
+struct Config {
+  int RB, TB, L;
+};
+
+static bool gate_diag_like(const qs_gate& g) { return g.kind == QS_GATE_DIAG || g.kind == QS_GATE_PHASE; }
+
+static uint64_t gate_mask(const qs_gate& g) {
+  uint64_t m = 0;
+  for (int i = 0; i < g.ncontrols + g.ntargets; ++i) m |= 1ull << g.bits[i];
+  return m;
+}
+
+// Upper bound of the coefficient entries a gate's op can need.
+static int coef_cost(const qs_gate& g, int NR) {
+  if (gate_diag_like(g)) return 4 * NR;
+  if (g.kind == QS_GATE_XPERM && g.ncontrols == 0) {  // a flip, plus a diagonal phase table unless pure
+    const bool pure = g.m[0] == 1.0 && g.m[1] == 0.0 && g.m[2] == 1.0 && g.m[3] == 0.0;
+    return pure ? 0 : 2 * NR;
+  }
+  if (g.kind == QS_GATE_XPERM) return 2;
+  if (g.kind == QS_GATE_SWAP) return 16;                         // may become a 4x4 under mixed flips
+  return 1 << (2 * g.ntargets);
+}
+
+static bool fusable(const qs_gate& g, const Config& c) {
+  const int K = c.RB + c.TB;
+  if (g.ncontrols + g.ntargets > K) return false;
+  if (g.kind == QS_GATE_DENSE && g.ntargets > std::min(4, c.RB)) return false;
+  if (gate_diag_like(g) && g.ncontrols + g.ntargets > c.RB + 2) return false;
+  return true;
+}
+
+// Complex payload of a gate converted to the op form.
+struct OpBuild {
+  FOp op;
+  std::vector<double> coef;  // (re, im) pairs
+};
+
+static void permute_matrix(const std::vector<double>& M, int k, const int* order, std::vector<double>& out) {
+  // order[i] = index of the original target that becomes new target i (new target 0 = MSB)
+  const int D = 1 << k;
+  out.assign(2 * D * D, 0.0);
+  auto remap = [&](int idx) {
+    int o = 0;  // original index
+    for (int i = 0; i < k; ++i) {
+      const int bit = (idx >> (k - 1 - i)) & 1;
+      o |= bit << (k - 1 - order[i]);
+    }
+    return o;
+  };
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      const int oi = remap(i), oj = remap(j);
+      out[2 * (i * D + j)] = M[2 * (oi * D + oj)];
+      out[2 * (i * D + j) + 1] = M[2 * (oi * D + oj) + 1];
+    }
+}
+
+// M'[i][j] = M[i ^ x][j ^ x]: conjugation by X on the flipped target bits.
+static void xor_conjugate(std::vector<double>& M, int k, int x) {
+  if (!x) return;
+  const int D = 1 << k;
+  std::vector<double> out(M.size());
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      out[2 * (i * D + j)] = M[2 * ((i ^ x) * D + (j ^ x))];
+      out[2 * (i * D + j) + 1] = M[2 * ((i ^ x) * D + (j ^ x)) + 1];
+    }
+  M.swap(out);
+}
+
+static int index_in(const std::vector<int>& v, int x) {
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == x) return (int)i;
+  return -1;
+}
+
+// Layout: which tile-local bit each register slot / thread slot carries, and
+// which slots are flipped (the slot's physical index bit is the complement of
+// its tile bit).  Slot numbering: register position p -> p, thread position q -> RB + q.
+struct Layout {
+  std::vector<int> rs;  // register bit i -> tile-local bit
+  std::vector<int> ts;  // thread bit i -> tile-local bit
+  uint32_t flip = 0;
+};
+
+static Layout make_layout(const std::vector<int>& regs, int K) {
+  Layout L;
+  L.rs = regs;
+  std::vector<char> isreg(K, 0);
+  for (int b : regs) isreg[b] = 1;
+  for (int b = 0; b < K; ++b)
+    if (!isreg[b]) L.ts.push_back(b);
+  return L;
+}
+
+static int slot_of(const Layout& L, int b) {
+  const int p = index_in(L.rs, b);
+  if (p >= 0) return p;
+  return (int)L.rs.size() + index_in(L.ts, b);
+}
+static int flipped(const Layout& L, int b) { return (L.flip >> slot_of(L, b)) & 1; }
+
+// Tile-local index mask of the flipped slots.
+static uint32_t flip_local(const Layout& L) {
+  uint32_t m = 0;
+  const int RB = (int)L.rs.size();
+  for (int s = 0; s < RB + (int)L.ts.size(); ++s)
+    if ((L.flip >> s) & 1) m |= 1u << (s < RB ? L.rs[s] : L.ts[s - RB]);
+  return m;
+}
+
+// Tile-local bits that must be register bits for gate g under the op encoding.
+static std::vector<int> needed_regs(const qs_gate& g, const std::vector<int>& loc, const Layout& cur) {
+  std::vector<int> need;
+  const int c = g.ncontrols, t = g.ntargets;
+  if (gate_diag_like(g)) {
+    // at most two of the gate's bits (controls or targets) may stay thread bits
+    int outside = 0;
+    for (int i = 0; i < c + t; ++i)
+      if (index_in(cur.rs, loc[i]) < 0) ++outside;
+    if (outside <= 2) return need;
+    for (int i = 0; i < t; ++i) need.push_back(loc[c + i]);
+    for (int i = 0; i < c && (int)need.size() < c + t - 2; ++i) need.push_back(loc[i]);
+    return need;
+  }
+  for (int i = 0; i < t; ++i) need.push_back(loc[c + i]);
+  return need;
+}
+
+// Diagonal table of a diagonal-like gate (or a plain per-bit diagonal given by
+// `vals`, 2^t entries over the targets) under layout `cur`.
+static int build_diag(const qs_gate& g, const std::vector<int>& loc, const Layout& cur, int NR, OpBuild& ob) {
+  FOp& op = ob.op;
+  const int c = g.ncontrols, t = g.ntargets;
+  std::vector<int> selb;  // gate-bit indices living on thread bits
+  for (int i = 0; i < c + t; ++i)
+    if (index_in(cur.rs, loc[i]) < 0) selb.push_back(i);
+  if (selb.size() > 2) return QS_ERR_UNSUPPORTED;
+  const int ns = (int)selb.size();
+  op.kind = F_DIAG;
+  op.ntt = ns;
+  for (int k = 0; k < ns; ++k) op.tpos[k] = index_in(cur.ts, loc[selb[k]]);
+  const int D = 1 << t;
+  ob.coef.assign(2 * (size_t)NR * (1 << ns), 0.0);
+  uint32_t mask = 0;
+  for (int sel = 0; sel < (1 << ns); ++sel)
+    for (int r = 0; r < NR; ++r) {
+      bool ctrl_ok = true;
+      int m = 0;
+      for (int i = 0; i < c + t; ++i) {
+        int bit;
+        const int si = index_in(selb, i);
+        if (si >= 0) bit = (sel >> (ns - 1 - si)) & 1;
+        else bit = (r >> index_in(cur.rs, loc[i])) & 1;
+        bit ^= flipped(cur, loc[i]);  // logical value of the gate bit
+        if (i < c) {
+          if (!bit) ctrl_ok = false;
+        } else if (bit) {
+          m |= 1 << (t - 1 - (i - c));
+        }
+      }
+      double re = 1.0, im = 0.0;
+      if (ctrl_ok) {
+        if (g.kind == QS_GATE_PHASE) {
+          if (m == D - 1) {
+            re = g.m[0];
+            im = g.m[1];
+          }
+        } else {
+          re = g.m[2 * m];
+          im = g.m[2 * m + 1];
+        }
+      }
+      const size_t e = (size_t)sel * NR + r;
+      ob.coef[2 * e] = re;
+      ob.coef[2 * e + 1] = im;
+      if (re != 1.0 || im != 0.0) mask |= 1u << r;
+    }
+  op.creg = mask;
+  return QS_OK;
+}
+
+static int build_op(const qs_gate& g, const std::vector<int>& loc, const Layout& cur, int NR, OpBuild& ob) {
+  FOp& op = ob.op;
+  std::memset(&op, 0, sizeof(op));
+  const int c = g.ncontrols, t = g.ntargets;
+  ob.coef.clear();
+  if (gate_diag_like(g)) return build_diag(g, loc, cur, NR, ob);
+  // controls of a non-diagonal gate: register-space mask + lane predicate,
+  // each with the physical value that means logical 1
+  for (int i = 0; i < c; ++i) {
+    const int b = loc[i];
+    const int one = 1 ^ flipped(cur, b);
+    const int ri = index_in(cur.rs, b);
+    if (ri >= 0) {
+      op.creg |= 1u << ri;
+      op.cregv |= (uint32_t)one << ri;
+    } else {
+      const int ti = index_in(cur.ts, b);
+      op.cthr |= 1u << ti;
+      op.cthrv |= (uint32_t)one << ti;
+    }
+  }
+  // non-diagonal: all targets are register bits
+  std::vector<int> rp(t);
+  for (int i = 0; i < t; ++i) {
+    rp[i] = index_in(cur.rs, loc[c + i]);
+    if (rp[i] < 0) return QS_ERR_UNSUPPORTED;
+  }
+  if (g.kind == QS_GATE_XPERM) {
+    op.kind = F_PERM1;
+    op.npos = 1;
+    op.pos[0] = rp[0];
+    double m0r = g.m[0], m0i = g.m[1], m1r = g.m[2], m1i = g.m[3];
+    if (flipped(cur, loc[c])) {  // physical 0 is logical 1: the phases trade places
+      std::swap(m0r, m1r);
+      std::swap(m0i, m1i);
+    }
+    op.pure = (m0r == 1.0 && m0i == 0.0 && m1r == 1.0 && m1i == 0.0);
+    ob.coef = {m0r, m0i, m1r, m1i};
+    return QS_OK;
+  }
+  std::vector<double> M;
+  if (g.kind == QS_GATE_SWAP) {
+    const int f0 = flipped(cur, loc[c]), f1 = flipped(cur, loc[c + 1]);
+    if (f0 == f1) {
+      op.kind = F_SWAP2;
+      op.npos = 2;
+      double m0r = g.m[0], m0i = g.m[1], m1r = g.m[2], m1i = g.m[3];
+      int p0 = rp[0], p1 = rp[1];
+      if (p0 < p1) {  // canonical: pos[0] > pos[1]; exchanging target roles swaps m0/m1
+        std::swap(p0, p1);
+        std::swap(m0r, m1r);
+        std::swap(m0i, m1i);
+      }
+      if (f0) {  // both flipped: physical 01 is logical 10
+        std::swap(m0r, m1r);
+        std::swap(m0i, m1i);
+      }
+      op.pos[0] = p0;
+      op.pos[1] = p1;
+      op.pure = (m0r == 1.0 && m0i == 0.0 && m1r == 1.0 && m1i == 0.0);
+      ob.coef = {m0r, m0i, m1r, m1i};
+      return QS_OK;
+    }
+    // one target flipped: the exchange becomes 00 <-> 11 physically; use the 4x4 form
+    M.assign(32, 0.0);
+    M[2 * 0] = 1.0;                                    // |00> -> |00>
+    M[2 * (1 * 4 + 2)] = g.m[0], M[2 * (1 * 4 + 2) + 1] = g.m[1];  // a01' = m0 a10
+    M[2 * (2 * 4 + 1)] = g.m[2], M[2 * (2 * 4 + 1) + 1] = g.m[3];  // a10' = m1 a01
+    M[2 * 15] = 1.0;                                   // |11> -> |11>
+  } else {
+    const int D = 1 << t;
+    M.resize(2 * D * D);
+    const double* src = (t <= 2) ? g.m : g.big;
+    for (int i = 0; i < 2 * D * D; ++i) M[i] = src[i];
+  }
+  // flips: conjugate by X on the flipped targets (gate target order, MSB first)
+  int x = 0;
+  for (int i = 0; i < t; ++i)
+    if (flipped(cur, loc[c + i])) x |= 1 << (t - 1 - i);
+  xor_conjugate(M, t, x);
+  // order targets by descending register position (MSB first)
+  std::vector<int> order(t);
+  for (int i = 0; i < t; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return rp[a] > rp[b]; });
+  std::vector<double> Mp;
+  permute_matrix(M, t, order.data(), Mp);
+  op.npos = t;
+  for (int i = 0; i < t; ++i) op.pos[i] = rp[order[i]];
+  op.kind = (t == 1) ? F_DENSE1 : (t == 2) ? F_DENSE2 : (t == 3) ? F_DENSE3 : F_DENSE4;
+  ob.coef = Mp;
+  return QS_OK;
+}
+
+// ---- peephole merging of built ops ---------------------------------------------
+//
+// Ops act on physical slots (register positions and thread positions); two ops
+// on disjoint slot sets commute.  A new op is merged into the most recent
+// earlier op that shares a slot with it (everything in between commutes with
+// it), never across a relayout:
+//   diag  x diag           -> one diagonal table (thread selector union <= 2 bits)
+//   dense1 x dense1        -> product matrix (same register position and controls)
+//   dense1 x diag on that register bit (no controls) -> product matrix
+// Relabelings (SWAP, X) move no data, so they never block a merge.
+
+struct Built {
+  FOp op;
+  std::vector<double> coef;  // (re, im) pairs
+  uint32_t slots = 0;        // physical footprint: bit p = register position p, bit RB+q = thread position q
+  std::vector<int> selp;     // DIAG: thread positions selecting the value, MSB first
+};
+
+using cplx = std::pair<double, double>;
+static inline cplx cm_(cplx a, cplx b) { return {a.first * b.first - a.second * b.second, a.first * b.second + a.second * b.first}; }
+
+static void mat2_mul(const double* A, const double* B, double* out) {  // out = A * B (2x2 complex)
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 2; ++k) {
+        const double ar = A[2 * (i * 2 + k)], ai = A[2 * (i * 2 + k) + 1];
+        const double br = B[2 * (k * 2 + j)], bi = B[2 * (k * 2 + j) + 1];
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      out[2 * (i * 2 + j)] = re;
+      out[2 * (i * 2 + j) + 1] = im;
+    }
+}
+
+// DIAG op confined to one register position: returns the position and (d0, d1), else -1.
+static int diag_single_reg(const Built& b, int NR, int RB, double* d2) {
+  if (b.op.kind != F_DIAG || !b.selp.empty() || __builtin_popcount(b.slots) != 1) return -1;
+  const int p = __builtin_ctz(b.slots);
+  if (p >= RB) return -1;
+  for (int r = 0; r < NR; ++r) {
+    const int r0 = r & (1 << p);
+    if (b.coef[2 * r] != b.coef[2 * r0] || b.coef[2 * r + 1] != b.coef[2 * r0 + 1]) return -1;
+  }
+  d2[0] = b.coef[0];
+  d2[1] = b.coef[1];
+  d2[2] = b.coef[2 * (1 << p)];
+  d2[3] = b.coef[2 * (1 << p) + 1];
+  return p;
+}
+
+static void diag_to_dense1(const double* d2, double* M) {
+  M[0] = d2[0]; M[1] = d2[1]; M[2] = 0; M[3] = 0;
+  M[4] = 0; M[5] = 0; M[6] = d2[2]; M[7] = d2[3];
+}
+
+static bool merge_diag(Built& e, const Built& d, int NR) {
+  std::vector<int> u = e.selp;
+  for (int b : d.selp)
+    if (index_in(u, b) < 0) u.push_back(b);
+  if (u.size() > 2) return false;
+  const int nu = (int)u.size();
+  auto proj = [&](const std::vector<int>& sb, int usel) {
+    int s = 0;
+    for (size_t k = 0; k < sb.size(); ++k) {
+      const int ui = index_in(u, sb[k]);
+      s = (s << 1) | ((usel >> (nu - 1 - ui)) & 1);
+    }
+    return s;
+  };
+  std::vector<double> T(2 * (size_t)NR * (1 << nu));
+  for (int us = 0; us < (1 << nu); ++us)
+    for (int r = 0; r < NR; ++r) {
+      const size_t ie = (size_t)proj(e.selp, us) * NR + r, id = (size_t)proj(d.selp, us) * NR + r;
+      const cplx v = cm_({e.coef[2 * ie], e.coef[2 * ie + 1]}, {d.coef[2 * id], d.coef[2 * id + 1]});
+      T[2 * ((size_t)us * NR + r)] = v.first;
+      T[2 * ((size_t)us * NR + r) + 1] = v.second;
+    }
+  e.coef.swap(T);
+  e.selp = u;
+  e.slots |= d.slots;
+  return true;
+}
+
+static bool uncontrolled(const FOp& o) { return o.creg == 0 && o.cthr == 0; }
+
+static bool try_merge(std::vector<Built>& out, Built& d, int NR, int RB) {
+  if (d.op.kind != F_DIAG && d.op.kind != F_DENSE1) return false;
+  int k = (int)out.size() - 1;
+  for (; k >= 0; --k) {
+    if (out[k].op.kind == F_RELAYOUT) return false;
+    if (out[k].slots & d.slots) break;
+  }
+  if (k < 0) return false;
+  Built& e = out[k];
+  if (d.op.kind == F_DIAG && e.op.kind == F_DIAG) return merge_diag(e, d, NR);
+  double d2[4], M[8];
+  if (d.op.kind == F_DENSE1 && e.op.kind == F_DENSE1 && e.op.pos[0] == d.op.pos[0] && e.op.creg == d.op.creg &&
+      e.op.cregv == d.op.cregv && e.op.cthr == d.op.cthr && e.op.cthrv == d.op.cthrv) {
+    mat2_mul(d.coef.data(), e.coef.data(), M);
+    e.coef.assign(M, M + 8);
+    return true;
+  }
+  if (d.op.kind == F_DENSE1 && uncontrolled(d.op)) {
+    const int p = diag_single_reg(e, NR, RB, d2);
+    if (p == d.op.pos[0]) {
+      double D[8];
+      diag_to_dense1(d2, D);
+      mat2_mul(d.coef.data(), D, M);
+      Built nb = d;
+      nb.coef.assign(M, M + 8);
+      out[k] = std::move(nb);
+      return true;
+    }
+    return false;
+  }
+  if (d.op.kind == F_DIAG && e.op.kind == F_DENSE1 && uncontrolled(e.op)) {
+    const int p = diag_single_reg(d, NR, RB, d2);
+    if (p == e.op.pos[0]) {
+      double D[8];
+      diag_to_dense1(d2, D);
+      mat2_mul(D, e.coef.data(), M);
+      e.coef.assign(M, M + 8);
+      return true;
+    }
+  }
+  return false;
+}
+
+// Final encoding of a DIAG op: register-independent tables collapse to one
+// value per selector (op.pure = 1, lane mode).
+static void finalize_diag(Built& b, int NR) {
+  const int ns = (int)b.selp.size();
+  b.op.ntt = ns;
+  for (int k = 0; k < ns; ++k) b.op.tpos[k] = b.selp[k];
+  bool lane = true;
+  uint32_t mask = 0;
+  for (int s = 0; s < (1 << ns); ++s)
+    for (int r = 0; r < NR; ++r) {
+      const size_t i = (size_t)s * NR + r, i0 = (size_t)s * NR;
+      if (b.coef[2 * i] != b.coef[2 * i0] || b.coef[2 * i + 1] != b.coef[2 * i0 + 1]) lane = false;
+      if (b.coef[2 * i] != 1.0 || b.coef[2 * i + 1] != 0.0) mask |= 1u << r;
+    }
+  if (lane) {
+    std::vector<double> T(2 * (1 << ns));
+    for (int s = 0; s < (1 << ns); ++s) {
+      T[2 * s] = b.coef[2 * (size_t)s * NR];
+      T[2 * s + 1] = b.coef[2 * (size_t)s * NR + 1];
+    }
+    b.coef.swap(T);
+    b.op.pure = 1;
+    b.op.creg = mask ? (uint32_t)((1ull << NR) - 1) : 0u;
+  } else {
+    b.op.pure = 0;
+    b.op.creg = mask;
+  }
+}
+
+struct PassPlan {
+  bool fused;
+  int single;                      // gate index when !fused
+  std::vector<int> gates;          // gate indices when fused
+  std::vector<int> tbits;          // tile-local -> global
+};
+
+static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c, std::vector<PassPlan>& plans) {
+  const int K = c.RB + c.TB;
+  std::vector<int> remaining(ngates);
+  for (int i = 0; i < ngates; ++i) remaining[i] = i;
+  const uint64_t low = (1ull << c.L) - 1;
+  while (!remaining.empty()) {
+    PassPlan p;
+    p.fused = true;
+    p.single = -1;
+    std::vector<int> rest;
+    size_t i = 0;
+    // A gate the tile kernel cannot take at the head of the list runs on its own.
+    if (!fusable(gates[remaining[0]], c)) {
+      p.fused = false;
+      p.single = remaining[0];
+      i = 1;
+    } else {
+      uint64_t S = low, blocked = 0;
+      int coef_budget = FMAX_PASS_COEF;  // LDS-staged payload per pass (complex entries)
+      for (; i < remaining.size(); ++i) {
+        const int gi = remaining[i];
+        const qs_gate& g = gates[gi];
+        const uint64_t gm = gate_mask(g);
+        const int cost = coef_cost(g, 1 << c.RB);
+        if ((gm & blocked) || !fusable(g, c) || __builtin_popcountll(S | gm) > K || cost > coef_budget) {
+          blocked |= gm;  // everything sharing a bit with a deferred gate must stay behind it
+          rest.push_back(gi);
+          continue;
+        }
+        S |= gm;
+        coef_budget -= cost;
+        p.gates.push_back(gi);
+      }
+      // fill the tile to K bits with the lowest unused bits: that lengthens the
+      // contiguous runs each workgroup reads and writes (DRAM page locality)
+      for (int b = 0; b < n && __builtin_popcountll(S) < K; ++b) S |= 1ull << b;
+      for (int b = 0; b < n; ++b)
+        if ((S >> b) & 1) p.tbits.push_back(b);  // ascending; the first L are the low bits
+    }
+    for (; i < remaining.size(); ++i) rest.push_back(remaining[i]);
+    plans.push_back(std::move(p));
+    remaining.swap(rest);
+  }
+}
+
+// Choose the register set for a relayout: `need` first, then bits needed soonest.
+static std::vector<int> choose_regs(const std::vector<int>& need, const std::vector<int>& lookahead,
+                                    const std::vector<int>& cur, int RB, int K, int L) {
+  std::vector<int> regs = need;
+  auto has = [&](int b) { return std::find(regs.begin(), regs.end(), b) != regs.end(); };
+  for (int b : lookahead) {
+    if ((int)regs.size() >= RB) break;
+    if (!has(b)) regs.push_back(b);
+  }
+  for (int b : cur) {
+    if ((int)regs.size() >= RB) break;
+    if (!has(b) && b >= L) regs.push_back(b);
+  }
+  for (int b = K - 1; b >= 0 && (int)regs.size() < RB; --b)
+    if (!has(b)) regs.push_back(b);
+  return regs;
+}
+
+// A whole pass program: headers, ops, complex coefficients and the relayout
+// constants, ready for upload.
+struct FusedProgram {
+  std::vector<PassPlan> plans;
+  std::vector<int> hdr_of;  // plan -> header (-1 for single-gate passes)
+  std::vector<PassHdr> hdrs;
+  std::vector<FOp> ops;
+  std::vector<double> coef;  // (re, im)
+  std::vector<uint32_t> aux;
+};
+
+// LDS offsets (bytes) of the slots of layout L: lr[p], lt[q].
+static void slot_offsets(const Layout& L, uint32_t amp_bytes, uint32_t* lr, uint32_t* lt) {
+  for (size_t p = 0; p < L.rs.size(); ++p) lr[p] = lds_swz(1u << L.rs[p]) * amp_bytes;
+  for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = lds_swz(1u << L.ts[q]) * amp_bytes;
+}
+
+static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& c, uint32_t amp_bytes, PassHdr& H,
+                         FusedProgram& prog) {
+  const int K = c.RB + c.TB;
+  std::memset(&H, 0, sizeof(H));
+  std::vector<int> g2l(64, -1);
+  for (int j = 0; j < K; ++j) {
+    H.tbits[j] = p.tbits[j];
+    H.sorted[j] = p.tbits[j];
+    g2l[p.tbits[j]] = j;
+  }
+  // per-gate local bits and lookahead list of needed (non-diagonal target) bits
+  std::vector<std::vector<int>> loc(p.gates.size());
+  for (size_t k = 0; k < p.gates.size(); ++k) {
+    const qs_gate& g = gates[p.gates[k]];
+    for (int i = 0; i < g.ncontrols + g.ntargets; ++i) loc[k].push_back(g2l[g.bits[i]]);
+  }
+  auto relabel_swap = [&](const qs_gate& g, const std::vector<int>& lb) {
+    // uncontrolled phase-free SWAP whose bits are both low or both high
+    return g.kind == QS_GATE_SWAP && g.ncontrols == 0 && g.m[0] == 1.0 && g.m[1] == 0.0 && g.m[2] == 1.0 &&
+           g.m[3] == 0.0 && ((lb[0] < c.L) == (lb[1] < c.L));
+  };
+  auto lookahead_from = [&](size_t k0) {
+    std::vector<int> la;
+    for (size_t k = k0; k < p.gates.size() && la.size() < 16; ++k) {
+      const qs_gate& g = gates[p.gates[k]];
+      if (gate_diag_like(g) || relabel_swap(g, loc[k])) continue;
+      if (g.kind == QS_GATE_XPERM && g.ncontrols == 0) continue;
+      for (int i = 0; i < g.ntargets; ++i) {
+        const int b = loc[k][g.ncontrols + i];
+        if (std::find(la.begin(), la.end(), b) == la.end()) la.push_back(b);
+      }
+    }
+    return la;
+  };
+  // initial layout: registers on high tile bits needed first
+  std::vector<int> init_need;
+  for (int b : lookahead_from(0)) {
+    if ((int)init_need.size() >= c.RB) break;
+    if (b >= c.L) init_need.push_back(b);
+  }
+  std::vector<int> regs = choose_regs(init_need, {}, {}, c.RB, K, c.L);
+  Layout cur = make_layout(regs, K);
+  for (int b = 0; b < c.RB; ++b) H.rs0[b] = cur.rs[b];
+  for (int b = 0; b < c.TB; ++b) H.ts0[b] = cur.ts[b];
+  const int op_start = (int)prog.ops.size();
+  const int coef_start = (int)(prog.coef.size() / 2);
+  const int NR = 1 << c.RB;
+  std::vector<Built> built;
+  auto relayout_to = [&](const std::vector<int>& nr) {
+    Built rl;
+    std::memset(&rl.op, 0, sizeof(rl.op));
+    rl.op.kind = F_RELAYOUT;
+    rl.op.coef = (int)prog.aux.size();
+    uint32_t lr[FMAX_RB], lt[FMAX_TB];
+    slot_offsets(cur, amp_bytes, lr, lt);
+    for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
+    for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
+    prog.aux.push_back(lds_swz(flip_local(cur)) * amp_bytes);
+    cur = make_layout(nr, K);
+    slot_offsets(cur, amp_bytes, lr, lt);
+    for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
+    for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
+    built.push_back(rl);
+    H.uses_lds = 1;
+  };
+  auto add_built = [&](OpBuild& ob, const std::vector<int>& lb) {
+    Built b;
+    b.op = ob.op;
+    b.coef = std::move(ob.coef);
+    for (int bit : lb) b.slots |= 1u << slot_of(cur, bit);
+    if (b.op.kind == F_DIAG)
+      for (int k = 0; k < b.op.ntt; ++k) b.selp.push_back(b.op.tpos[k]);
+    if (!try_merge(built, b, NR, c.RB)) built.push_back(std::move(b));
+  };
+  for (size_t k = 0; k < p.gates.size(); ++k) {
+    const qs_gate& g = gates[p.gates[k]];
+    if (relabel_swap(g, loc[k])) {
+      // the two slots exchange the tile bits they carry; flips stay with the slots
+      const int a = loc[k][0], b = loc[k][1];
+      auto& va = (index_in(cur.rs, a) >= 0) ? cur.rs : cur.ts;
+      auto& vb = (index_in(cur.rs, b) >= 0) ? cur.rs : cur.ts;
+      const int ia = index_in(va, a), ib = index_in(vb, b);
+      va[ia] = b;
+      vb[ib] = a;
+      continue;
+    }
+    if (g.kind == QS_GATE_XPERM && g.ncontrols == 0) {
+      // flip the slot; a0' = m0 a1, a1' = m1 a0 leaves diag(m0, m1) on the flipped bit
+      const int b = loc[k][0];
+      cur.flip ^= 1u << slot_of(cur, b);
+      if (!(g.m[0] == 1.0 && g.m[1] == 0.0 && g.m[2] == 1.0 && g.m[3] == 0.0)) {
+        qs_gate d;
+        std::memset(&d, 0, sizeof(d));
+        d.kind = QS_GATE_DIAG;
+        d.ntargets = 1;
+        d.bits[0] = g.bits[0];
+        d.m[0] = g.m[0], d.m[1] = g.m[1], d.m[2] = g.m[2], d.m[3] = g.m[3];
+        OpBuild ob;
+        std::memset(&ob.op, 0, sizeof(ob.op));
+        int rc = build_diag(d, loc[k], cur, NR, ob);
+        if (rc) {
+          set_error("fusion planner produced an op the tile kernel cannot encode");
+          return rc;
+        }
+        add_built(ob, loc[k]);
+      }
+      continue;
+    }
+    std::vector<int> need = needed_regs(g, loc[k], cur);
+    bool ok = true;
+    for (int b : need)
+      if (index_in(cur.rs, b) < 0) ok = false;
+    if (!ok) relayout_to(choose_regs(need, lookahead_from(k + 1), cur.rs, c.RB, K, c.L));
+    OpBuild ob;
+    int rc = build_op(g, loc[k], cur, NR, ob);
+    if (rc) {
+      set_error("fusion planner produced an op the tile kernel cannot encode");
+      return rc;
+    }
+    add_built(ob, loc[k]);
+  }
+  // final layout must keep the low L bits on threads
+  bool low_in_regs = false;
+  for (int b : cur.rs)
+    if (b < c.L) low_in_regs = true;
+  if (low_in_regs) {
+    std::vector<int> keep;
+    for (int b : cur.rs)
+      if (b >= c.L) keep.push_back(b);
+    relayout_to(choose_regs(keep, {}, {}, c.RB, K, c.L));
+  }
+  for (Built& b : built) {
+    if (b.op.kind == F_DIAG) {
+      finalize_diag(b, NR);
+      if (b.op.creg == 0) continue;  // the merged product is the identity
+    }
+    if (b.op.kind == F_DENSE3 || b.op.kind == F_DENSE4) H.has_dk = 1;
+    if (b.op.kind != F_RELAYOUT) {
+      b.op.coef = (int)(prog.coef.size() / 2) - coef_start;
+      prog.coef.insert(prog.coef.end(), b.coef.begin(), b.coef.end());
+    }
+    prog.ops.push_back(b.op);
+  }
+  for (int b = 0; b < c.RB; ++b) H.rs1[b] = cur.rs[b];
+  for (int b = 0; b < c.TB; ++b) H.ts1[b] = cur.ts[b];
+  // pending flips go into the store address
+  const uint32_t fl = flip_local(cur);
+  for (int j = 0; j < K; ++j)
+    if ((fl >> j) & 1) H.sflip |= 1ull << p.tbits[j];
+  bool same = H.sflip == 0;
+  for (int b = 0; b < c.RB; ++b) same = same && H.rs1[b] == H.rs0[b];
+  for (int b = 0; b < c.TB; ++b) same = same && H.ts1[b] == H.ts0[b];
+  H.sync_store = (!H.uses_lds && !same) ? 1 : 0;
+  H.op0 = op_start;
+  H.nops = (int)prog.ops.size() - op_start;
+  H.coef0 = coef_start;
+  H.ncoef = (int)(prog.coef.size() / 2) - coef_start;
+  return QS_OK;
+}
+
+static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
+                         FusedProgram& prog) {
+  plan_groups(gates, ngates, n, c, prog.plans);
+  prog.hdr_of.assign(prog.plans.size(), -1);
+  for (size_t i = 0; i < prog.plans.size(); ++i) {
+    if (!prog.plans[i].fused) continue;
+    PassHdr H;
+    int rc = schedule_pass(gates, prog.plans[i], c, amp_bytes, H, prog);
+    if (rc) return rc;
+    prog.hdr_of[i] = (int)prog.hdrs.size();
+    prog.hdrs.push_back(H);
+  }
+  return QS_OK;
+}
+
+static void print_program_stats(const FusedProgram& prog, int n, int K, int L, int ngates) {
+  int nrel = 0, kinds[8] = {0};
+  for (const FOp& o : prog.ops) {
+    if (o.kind == F_RELAYOUT) ++nrel;
+    if (o.kind < 8) ++kinds[o.kind];
+  }
+  int lds = 0;
+  for (auto& h : prog.hdrs) lds += h.uses_lds;
+  fprintf(stderr,
+          "[qsim fusion] n=%d K=%d L=%d gates=%d passes=%zu fused=%zu with_lds=%d ops=%zu relayouts=%d "
+          "dense1=%d perm1=%d diag=%d dense2=%d swap2=%d dense3=%d dense4=%d\n",
+          n, K, L, ngates, prog.plans.size(), prog.hdrs.size(), lds, prog.ops.size(), nrel, kinds[F_DENSE1],
+          kinds[F_PERM1], kinds[F_DIAG], kinds[F_DENSE2], kinds[F_SWAP2], kinds[F_DENSE3], kinds[F_DENSE4]);
+}
+
+}  // namespace qs

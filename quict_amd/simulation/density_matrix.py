@@ -125,8 +125,26 @@ class DensityMatrixSimulator:
             self.last_passes += passes.value
             recs.clear()
 
+        def unitary(g):
+            row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+            col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+            if compile_gate(g, n, row, keep, row_offset=n):
+                compile_gate(g, n, col, keep, row_offset=0, conj=True)
+                recs.append(row)
+                recs.append(col)
+
         for g in circuit.gates:
             if isinstance(g, NoiseGate):
+                factors = g.source[1].factors if g.source is not None else None
+                if factors and len(factors) > 1 and len(factors) == len(g.targs) and \
+                        all(f.qubits == 1 for f in factors):
+                    # K = (A (x) B ...) U: apply U, then each single-qubit factor channel
+                    unitary(g.source[0])
+                    for f, q in zip(factors, g.targs):
+                        rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+                        superop_record(f.kraus, [q], n, rec, keep)
+                        recs.append(rec)
+                    continue
                 rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
                 superop_record(g.noise_matrix, list(g.targs), n, rec, keep)
                 recs.append(rec)
@@ -136,12 +154,7 @@ class DensityMatrixSimulator:
             elif isinstance(g, BasicGate) and g.type in (GateType.reset, GateType.perm):
                 raise qerr.GateTypeNotImplementError(f"{g.type.name} in the density-matrix simulator")
             elif isinstance(g, BasicGate):
-                row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-                col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-                if compile_gate(g, n, row, keep, row_offset=n):
-                    compile_gate(g, n, col, keep, row_offset=0, conj=True)
-                    recs.append(row)
-                    recs.append(col)
+                unitary(g)
             else:
                 raise qerr.TypeError("DensityMatrixSimulator takes BasicGate / NoiseGate, got", type(g))
         flush()

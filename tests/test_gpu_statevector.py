@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from quict_amd.core import (Circuit, GateType, H, CX, Ry, Rz, X, Measure, Reset, Perm, Unitary, QFT, IQFT, CCX,
-                            CSwap, Rzx, FSim)
+                            CSwap, Rzx, FSim, Swap)
 from quict_amd.core import qasm
 from quict_amd import workloads
 from oracle import statevector as osv
@@ -120,6 +120,29 @@ def test_fused_equals_unfused_26q_and_inverse_roundtrip():
     e0 = np.zeros(1 << n, dtype=np.complex128)
     e0[0] = 1
     assert np.max(np.abs(back - e0)) < 1e-11
+
+
+def test_relabel_and_flip_passes_are_exact():
+    """Uncontrolled X and Swap gates compile to relabelings of the tile layout
+    (no arithmetic): the load and store layouts of such passes differ.  Every
+    amplitude of the prepared state has its own phase, so a misplaced amplitude
+    shows as an O(1e-4) error.  Many workgroups per pass (24 qubits) so that the
+    waves of one workgroup run out of step."""
+    say("relabel / flip passes 24q")
+    n = 24
+    rng = random.Random(17)
+    c = Circuit(n)
+    for q in range(n):
+        H | c(q)
+        Rz(0.05 + 0.11 * q) | c(q)
+    for _ in range(200):
+        if rng.random() < 0.5:
+            X | c(rng.randrange(n))
+        else:
+            Swap | c(rng.sample(range(n), 2))
+    va = sim("double", True).run(c).get()
+    vb = sim("double", False).run(c).get()
+    assert np.max(np.abs(va - vb)) < 1e-12
 
 
 def test_measure_seeded_matches_oracle():

@@ -186,3 +186,35 @@ def test_channel_closed_forms():
     ng = NoiseGate(damp.kraus, 1)
     vec = ng.superoperator() @ rho.ravel()
     assert np.allclose(vec.reshape(2, 2), odm.amp_damp_closed_form(rho, 0.2), atol=1e-14)
+
+
+def test_tensor_channel_factors_compose_to_the_channel():
+    """A tensored channel after a two-qubit gate equals the gate, then the first
+    factor on the first argument, then the second on the second (the form the
+    density-matrix simulator applies).  Different damping strengths on the two
+    factors make a swapped order fail."""
+    from quict_amd.core import CRz
+    rng = np.random.RandomState(7)
+    A = rng.randn(4, 4) + 1j * rng.randn(4, 4)
+    rho = A @ A.conj().T
+    rho /= np.trace(rho)
+    d1, d2 = DampingError(0.05, 0, 0), DampingError(0.4, 0, 0)
+    t = d1.tensor(d2)
+    assert t.factors == [d1, d2]
+    assert d1.factors == [d1]
+    U = CRz(0.7).matrix.astype(np.complex128)
+    whole = sum(k @ rho @ k.conj().T for k in t.apply_to_gate(U))
+    step = U @ rho @ U.conj().T
+    k1 = [np.kron(k, np.identity(2)) for k in d1.kraus]
+    k2 = [np.kron(np.identity(2), k) for k in d2.kraus]
+    step = sum(k @ step @ k.conj().T for k in k1)
+    step = sum(k @ step @ k.conj().T for k in k2)
+    assert np.max(np.abs(whole - step)) < 1e-14
+    # the per-qubit closed forms agree with the factor order too
+    nm = NoiseModel(basic_gates=["crz"])
+    nm.add_noise_for_all_qubits(t, ["crz"])
+    c = Circuit(2)
+    CRz(0.7) | c([0, 1])
+    noisy = nm.transpile(c, accumulated_mode=True)
+    ng = noisy.gates[0]
+    assert ng.source[1] is t and ng.source[0].type == GateType.crz

@@ -1,0 +1,158 @@
+"""Fusion planner on the CPU.
+
+The planner (quict_amd/csrc/qsim_fused_plan.hpp) turns a gate list into pass
+programs: tile bits, register/thread layouts, relabelings (SWAP), flips (X, Y),
+LDS relayouts with XOR-swizzled slot offsets, merged ops and coefficient
+tables.  tests/planner/emu.cpp runs those exact programs with the semantics of
+the device interpreter (per tile, per lane and register, LDS image with range
+and collision checks).  Its results must match the C restatement of the
+reference kernels (oracle/csrc/cpu_ref_sim.c, itself pinned against the numpy
+oracle and the reference's golden vector) to 1e-12, starting from a random
+state so that every amplitude position matters.  The GPU tests run the same
+programs on the device.
+"""
+import ctypes
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+from quict_amd import _native as nat
+from quict_amd import workloads
+from quict_amd.core import (Circuit, H, Unitary, QFT, IQFT, CCX, CSwap, Rzx, FSim, X, Y, Swap, CX, CY)
+from quict_amd.simulation.gate_compiler import compile_gates
+from oracle import statevector as osv
+from test_oracle import ALL_TYPES
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "planner", "_build", "libplanner_emu.so")
+C128 = (4, 8, 16)  # register bits, thread bits, LDS bytes per amplitude
+C64 = (4, 9, 8)
+
+
+@pytest.fixture(scope="module")
+def emu():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "planner")], check=True)
+    lib = ctypes.CDLL(LIB)
+    lib.emu_apply.restype = ctypes.c_int
+    lib.emu_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.cpu_apply_gates.restype = ctypes.c_int
+    lib.emu_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def run_both(lib, c, n, shape, L=4, seed=0):
+    recs, keep = compile_gates(c.gates, n)
+    rng = np.random.RandomState(seed)
+    v = rng.randn(1 << n) + 1j * rng.randn(1 << n)
+    v /= np.linalg.norm(v)
+    ref = v.copy()
+    lib.cpu_apply_gates(ref.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 1)
+    out = v.copy()
+    st = np.zeros(16, dtype=np.int32)
+    RB, TB, S = shape
+    rc = lib.emu_apply(out.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), RB, TB, L, S,
+                       st.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0, lib.emu_last_error()
+    return out, ref, st
+
+
+def all_types(n, ngates, seed):
+    c = Circuit(n)
+    random.seed(seed)
+    np.random.seed(seed)
+    c.random_append(ngates, typelist=ALL_TYPES, random_params=True)
+    return c
+
+
+def flips_and_swaps(n, ngates, seed):
+    rng = random.Random(seed)
+    np.random.seed(seed)
+    c = Circuit(n)
+    for _ in range(ngates):
+        k = rng.random()
+        if k < 0.25:
+            X | c(rng.randrange(n))
+        elif k < 0.4:
+            Y | c(rng.randrange(n))
+        elif k < 0.6:
+            Swap | c(rng.sample(range(n), 2))
+        elif k < 0.7:
+            CX | c(rng.sample(range(n), 2))
+        elif k < 0.75:
+            CY | c(rng.sample(range(n), 2))
+        elif k < 0.8:
+            CSwap | c(rng.sample(range(n), 3))
+        else:
+            c.random_append(1, typelist=ALL_TYPES, random_params=True)
+    return c
+
+
+def dense_multi(n=16):
+    rng = np.random.RandomState(4)
+    c = Circuit(n)
+    for q in range(n):
+        H | c(q)
+    for k, args in [(3, [5, 1, 12]), (4, [0, 15, 7, 3]), (5, [2, 9, 14, 6, 11]), (3, [13, 14, 15])]:
+        q, _ = np.linalg.qr(rng.randn(1 << k, 1 << k) + 1j * rng.randn(1 << k, 1 << k))
+        Unitary(q) | c(args)
+        Rzx(0.3 * k) | c([args[0], args[1]])
+    QFT(4) | c([6, 2, 0, 4])
+    IQFT(3) | c([8, 10, 1])
+    CCX | c([3, 7, 11])
+    CSwap | c([0, 12, 4])
+    FSim(0.4, 0.9) | c([9, 2])
+    return c
+
+
+@pytest.mark.parametrize("shape", [C128, C64], ids=["c128-tile", "c64-tile"])
+@pytest.mark.parametrize("n,ngates,seed", [(14, 200, 1), (15, 300, 2), (16, 400, 3)])
+def test_all_gate_types(emu, shape, n, ngates, seed):
+    out, ref, st = run_both(emu, all_types(n, ngates, seed), n, shape)
+    assert st[1] > 0  # fused passes were planned
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [C128, C64], ids=["c128-tile", "c64-tile"])
+@pytest.mark.parametrize("n,ngates,seed", [(14, 300, 5), (16, 500, 6)])
+def test_relabelings_and_flips(emu, shape, n, ngates, seed):
+    out, ref, st = run_both(emu, flips_and_swaps(n, ngates, seed), n, shape)
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [C128, C64], ids=["c128-tile", "c64-tile"])
+def test_multi_target_dense(emu, shape):
+    out, ref, st = run_both(emu, dense_multi(), 16, shape)
+    assert st[3 + 6] + st[3 + 7] > 0  # 3- and 4-target in-register ops were used
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("L", [2, 5, 6])
+def test_low_bit_counts(emu, L):
+    c = workloads.random_mix_circuit(16, 300, 21, workloads.RANDOM_GATE_PROBS_60_40)
+    out, ref, st = run_both(emu, c, 16, C128, L=L)
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
+def test_headline_generator_at_18_qubits(emu):
+    """The 2000-gate 60/40 generator of the 30-qubit workload, at 18 qubits."""
+    out, ref, st = run_both(emu, workloads.sv30_circuit(n=18), 18, C128)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    assert st[1] < 120  # about twenty gates per pass
+
+
+def test_from_zero_state_vs_numpy_oracle(emu):
+    n = 14
+    c = all_types(n, 250, 9)
+    recs, keep = compile_gates(c.gates, n)
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1
+    st = np.zeros(16, dtype=np.int32)
+    rc = emu.emu_apply(v.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 4, 8, 4, 16,
+                       st.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    ref = osv.run(all_types(n, 250, 9))
+    assert np.max(np.abs(v - ref)) < 1e-12
