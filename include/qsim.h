@@ -106,8 +106,9 @@ int qs_sv_apply_gate(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* ga
 int qs_sv_apply_gates(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
                       int fuse, int* passes, void* stream);
 
-/* Fusion tile shape: log2 amplitudes per tile (0 = built-in default) and the
-   number of low bits always kept contiguous.  Set by the autotuner. */
+/* Fusion tile shape: log2 amplitudes per tile (0 = built-in default 12; also
+   11 or 13 for complex128, 13 for complex64) and the number of low bits always
+   kept contiguous (0 = 4, or 2..8). */
 int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
 
 /* Launch shape of the fused tile pass: prefetch != 0 selects persistent

@@ -331,7 +331,7 @@ uint64_t qs_ctx_scratch_bytes(const qs_ctx* ctx) { return ctx ? ctx->scratch_byt
 
 int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits) {
   QS_REQUIRE(ctx, "null ctx");
-  QS_REQUIRE(tile_log2 == 0 || (tile_log2 >= 8 && tile_log2 <= 14), "tile_log2 must be 0 or in [8, 14]");
+  QS_REQUIRE(tile_log2 == 0 || (tile_log2 >= 11 && tile_log2 <= 13), "tile_log2 must be 0 or in [11, 13]");
   QS_REQUIRE(low_bits == 0 || (low_bits >= 2 && low_bits <= 8), "low_bits must be 0 or in [2, 8]");
   ctx->tile_log2 = tile_log2;
   ctx->low_bits = low_bits;

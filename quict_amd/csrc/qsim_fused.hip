@@ -255,6 +255,35 @@ __device__ __forceinline__ void do_diag(cx<R> (&a)[NR], uint32_t creg, uint32_t 
     if ((creg >> r) & 1u) a[r] = mul(ld(Tl + r), a[r]);
 }
 
+// Relayout in two rounds through a half-tile LDS image (see the RELAYOUT
+// payload): round k moves the registers whose bit P is k, writing them with the
+// current layout (pending flips included) and reading the same registers back
+// with the new layout.
+template <typename R, int RB, int TB, int P>
+__device__ __forceinline__ void relayout_half(cx<R> (&a)[1 << RB], char* lds, const uint32_t* __restrict__ x,
+                                              uint32_t tid) {
+  constexpr int NR = 1 << RB;
+  uint32_t lw[RB], lq[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) lw[b] = x[1 + b];
+  const uint32_t w = thr_off<TB>(x + 1 + RB, tid) ^ x[1 + RB + TB];
+  const uint32_t* y = x + 2 + RB + TB;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) lq[b] = y[b];
+  const uint32_t rd = thr_off<TB>(y + RB, tid);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (((r >> P) & 1) == k) st_lds<R>(lds, w ^ reg_off<RB>(r, lw), a[r]);
+    lds_barrier();
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (((r >> P) & 1) == k) a[r] = ld_lds<R>(lds, rd ^ reg_off<RB>(r, lq));
+  }
+}
+
 // Apply a pass's op list to the register tile `a` (load layout on entry, store
 // layout on exit).  DK: include the in-register 3- and 4-target dense ops.
 template <typename R, int RB, int TB, bool DK>
@@ -274,22 +303,12 @@ __device__ __forceinline__ void run_ops(cx<R> (&a)[1 << RB], char* lds, const Pa
         flush_lane<R, NR>(a, f);
         fdirty = false;
       }
-      // write with the current layout (pending flips included), read with the new one
       const uint32_t* x = X + opcoef;
-      uint32_t lr[RB];
-#pragma unroll
-      for (int b = 0; b < RB; ++b) lr[b] = x[b];
-      const uint32_t w = thr_off<TB>(x + RB, tid) ^ x[RB + TB];
-      lds_barrier();
-#pragma unroll
-      for (int r = 0; r < NR; ++r) st_lds<R>(lds, w ^ reg_off<RB>(r, lr), a[r]);
-      lds_barrier();
-      const uint32_t* y = x + RB + TB + 1;
-#pragma unroll
-      for (int b = 0; b < RB; ++b) lr[b] = y[b];
-      const uint32_t rd = thr_off<TB>(y + RB, tid);
-#pragma unroll
-      for (int r = 0; r < NR; ++r) a[r] = ld_lds<R>(lds, rd ^ reg_off<RB>(r, lr));
+      const uint32_t pk = x[0];
+      if (pk == 0) relayout_half<R, RB, TB, 0>(a, lds, x, tid);
+      else if (pk == 1) relayout_half<R, RB, TB, 1>(a, lds, x, tid);
+      else if (pk == 2 || RB < 4) relayout_half<R, RB, TB, 2>(a, lds, x, tid);
+      else relayout_half<R, RB, TB, (RB > 3 ? 3 : 0)>(a, lds, x, tid);
       continue;
     }
     const int pure = (int)((e.w[0] >> 8) & 0xff);
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
   if constexpr (!DK) {
     // Stage the pass coefficients in LDS once per workgroup: op payload reads are
     // then LDS reads, which never wait on the global tile loads.
-    cx<R>* lc = reinterpret_cast<cx<R>*>(lds_raw) + (H->uses_lds ? (1 << K) : 0);
+    cx<R>* lc = reinterpret_cast<cx<R>*>(lds_raw) + (H->uses_lds ? (1 << (K - 1)) : 0);
     const cx<R>* gc = coef + H->coef0;
     const int nc = H->ncoef;
     for (int i = (int)tid; i < nc; i += (int)blockDim.x) lc[i] = gc[i];
@@ -519,7 +538,7 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
   const uint32_t* dX = (const uint32_t*)(dprog + hb + ob);
   const cx<R>* dC = (const cx<R>*)(dprog + hb + ob + xb);
   const uint64_t tiles = 1ull << (n - K);
-  const size_t lds_bytes = (size_t)(1u << K) * sizeof(cx<R>);
+  const size_t lds_bytes = (size_t)(1u << (K - 1)) * sizeof(cx<R>);  // half-tile relayout image
   int npass = 0;
   for (size_t i = 0; i < prog.plans.size(); ++i) {
     if (!prog.plans[i].fused) {
@@ -564,7 +583,7 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
 template <typename R, int RB, int TB> static void set_lds_attr() {
   static bool done = false;
   if (done) return;
-  const int bytes = (int)((1u << (RB + TB)) * sizeof(cx<R>) + FMAX_PASS_COEF * sizeof(cx<R>));
+  const int bytes = (int)((1u << (RB + TB - 1)) * sizeof(cx<R>) + FMAX_PASS_COEF * sizeof(cx<R>));
   (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB, true, false>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   (void)hipFuncSetAttribute((const void*)k_fused<R, RB, TB, false, true>,
@@ -576,10 +595,12 @@ template <typename R, int RB, int TB> static void set_lds_attr() {
 
 int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int* passes,
                       hipStream_t s) {
-  // Tile shapes: complex128 K = 12 (4 register bits x 256 threads, 64 KiB LDS);
-  // complex64 K = 13 (4 x 512, 64 KiB).
-  const int K = (dtype == QS_C128) ? 12 : 13;
-  int L = ctx->low_bits ? ctx->low_bits : 4;
+  // Tile shapes (4 register bits per thread, 2^(K-4) threads, a half-tile LDS
+  // image for relayouts): K = 12 by default for both precisions (256 threads;
+  // 32 KiB of LDS for complex128, 16 KiB for complex64); complex128 also takes
+  // 11 or 13, complex64 13.
+  const int K = ctx->tile_log2 ? ctx->tile_log2 : 12;
+  const int L = ctx->low_bits ? ctx->low_bits : 4;
   if (n < K + 1 || ngates == 0) {
     // small registers: every pass is latency-bound anyway; one kernel per gate
     for (int i = 0; i < ngates; ++i) {
@@ -590,11 +611,34 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
     return QS_OK;
   }
   if (dtype == QS_C128) {
-    set_lds_attr<double, 4, 8>();
-    return run_fused_t<double, 4, 8>(ctx, (cx<double>*)vec, n, gates, ngates, L, passes, s);
+    cx<double>* v = (cx<double>*)vec;
+    switch (K) {
+      case 11:
+        set_lds_attr<double, 4, 7>();
+        return run_fused_t<double, 4, 7>(ctx, v, n, gates, ngates, L, passes, s);
+      case 12:
+        set_lds_attr<double, 4, 8>();
+        return run_fused_t<double, 4, 8>(ctx, v, n, gates, ngates, L, passes, s);
+      case 13:
+        set_lds_attr<double, 4, 9>();
+        return run_fused_t<double, 4, 9>(ctx, v, n, gates, ngates, L, passes, s);
+      default:
+        set_error("complex128 fusion tiles: tile_log2 must be 11, 12 or 13");
+        return QS_ERR_UNSUPPORTED;
+    }
   }
-  set_lds_attr<float, 4, 9>();
-  return run_fused_t<float, 4, 9>(ctx, (cx<float>*)vec, n, gates, ngates, L, passes, s);
+  cx<float>* v = (cx<float>*)vec;
+  switch (K) {
+    case 12:
+      set_lds_attr<float, 4, 8>();
+      return run_fused_t<float, 4, 8>(ctx, v, n, gates, ngates, L, passes, s);
+    case 13:
+      set_lds_attr<float, 4, 9>();
+      return run_fused_t<float, 4, 9>(ctx, v, n, gates, ngates, L, passes, s);
+    default:
+      set_error("complex64 fusion tiles: tile_log2 must be 12 or 13");
+      return QS_ERR_UNSUPPORTED;
+  }
 }
 
 }  // namespace qs

@@ -67,13 +67,15 @@ static FOpW encode_op(const FOp& o) {
   return e;
 }
 
-// RELAYOUT payload in the aux table (uint32): the RB register-slot and TB
-// thread-slot LDS offsets of the current layout, the flip term of the current
-// layout, then the RB + TB slot offsets of the new layout.  An offset is the
-// byte offset of the XOR-swizzled LDS position of the single tile bit the slot
-// carries; the swizzle is XOR-linear, so an amplitude's LDS address is the XOR
-// of its slots' offsets.  The write side is carried per relayout because
-// relabelings change the layout between relayouts.
+// RELAYOUT payload in the aux table (uint32): the register position p of the
+// tile bit h that splits the exchange into two rounds; the RB register-slot and
+// TB thread-slot LDS offsets of the current layout; the flip term of the
+// current layout; then the RB + TB slot offsets of the new layout.  Offsets
+// address a half-tile image indexed by the tile bits other than h: an offset is
+// the byte offset of the XOR-swizzled position of the single tile bit its slot
+// carries (0 for the slot of h); the swizzle is XOR-linear, so an amplitude's
+// LDS address is the XOR of its slots' offsets.  The write side is carried per
+// relayout because relabelings change the layout between relayouts.
 struct PassHdr {
   int32_t nops;
   int32_t op0;
@@ -620,10 +622,16 @@ struct FusedProgram {
   std::vector<uint32_t> aux;
 };
 
-// LDS offsets (bytes) of the slots of layout L: lr[p], lt[q].
-static void slot_offsets(const Layout& L, uint32_t amp_bytes, uint32_t* lr, uint32_t* lt) {
-  for (size_t p = 0; p < L.rs.size(); ++p) lr[p] = lds_swz(1u << L.rs[p]) * amp_bytes;
-  for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = lds_swz(1u << L.ts[q]) * amp_bytes;
+
+// Tile-local index with bit h removed (higher bits shift down by one).
+static uint32_t drop_bit(uint32_t m, int h) { return (m & ((1u << h) - 1)) | ((m >> (h + 1)) << h); }
+
+// LDS offsets (bytes) of the slots of layout L in the half-tile image that
+// leaves out tile bit h; the slot carrying h gets offset 0.
+static void slot_offsets_half(const Layout& L, int h, uint32_t amp_bytes, uint32_t* lr, uint32_t* lt) {
+  auto off = [&](int b) -> uint32_t { return b == h ? 0u : lds_swz(drop_bit(1u << b, h)) * amp_bytes; };
+  for (size_t p = 0; p < L.rs.size(); ++p) lr[p] = off(L.rs[p]);
+  for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = off(L.ts[q]);
 }
 
 static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& c, uint32_t amp_bytes, PassHdr& H,
@@ -674,22 +682,46 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   const int coef_start = (int)(prog.coef.size() / 2);
   const int NR = 1 << c.RB;
   std::vector<Built> built;
-  auto relayout_to = [&](const std::vector<int>& nr) {
+  // Relayout through a half-tile LDS image: a tile bit h that is a register bit
+  // of both layouts, at the same register position p, splits the exchange into
+  // two rounds (registers with bit p = 0, then 1); each round writes 2^(K-1)
+  // amplitudes and reads them back into the registers it just wrote, so the
+  // LDS image needs only half the tile.  A flip of slot p carries over.
+  auto relayout_one = [&](std::vector<int> nr) {
+    int p = -1;
+    for (int i = 0; i < c.RB && p < 0; ++i)
+      if (index_in(nr, cur.rs[i]) >= 0) p = i;
+    const int h = cur.rs[p];
+    std::swap(nr[index_in(nr, h)], nr[p]);
     Built rl;
     std::memset(&rl.op, 0, sizeof(rl.op));
     rl.op.kind = F_RELAYOUT;
     rl.op.coef = (int)prog.aux.size();
     uint32_t lr[FMAX_RB], lt[FMAX_TB];
-    slot_offsets(cur, amp_bytes, lr, lt);
+    prog.aux.push_back((uint32_t)p);
+    slot_offsets_half(cur, h, amp_bytes, lr, lt);
     for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
     for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
-    prog.aux.push_back(lds_swz(flip_local(cur)) * amp_bytes);
+    prog.aux.push_back(lds_swz(drop_bit(flip_local(cur), h)) * amp_bytes);
+    const uint32_t fp = (cur.flip >> p) & 1u;
     cur = make_layout(nr, K);
-    slot_offsets(cur, amp_bytes, lr, lt);
+    cur.flip = fp << p;
+    slot_offsets_half(cur, h, amp_bytes, lr, lt);
     for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
     for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
     built.push_back(rl);
     H.uses_lds = 1;
+  };
+  auto relayout_to = [&](const std::vector<int>& nr) {
+    bool shared = false;
+    for (int b : cur.rs) shared = shared || index_in(nr, b) >= 0;
+    if (!shared) {
+      // no common register bit: go through a layout that keeps one old register bit
+      std::vector<int> mid(nr.begin(), nr.end() - 1);
+      mid.push_back(cur.rs[0]);
+      relayout_one(mid);
+    }
+    relayout_one(nr);
   };
   auto add_built = [&](OpBuild& ob, const std::vector<int>& lb) {
     Built b;

@@ -61,7 +61,6 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
           if ((r >> b) & 1) g |= 1ull << H.tbits[H.rs0[b]];
         a[(size_t)tid * NR + r] = v[tb | gin[tid] | g];
       }
-    std::vector<uint32_t> lr(RB), lt(TB);
     for (int tid = 0; tid < NT; ++tid) f[tid] = 1.0;
     bool fdirty = false;
     auto flush = [&]() {
@@ -71,41 +70,41 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
       }
       fdirty = false;
     };
-    auto toff = [&](int tid) {
-      uint32_t o = 0;
-      for (int b = 0; b < TB; ++b)
-        if ((tid >> b) & 1) o ^= lt[b];
-      return o;
-    };
-    auto roff = [&](int r) {
-      uint32_t o = 0;
-      for (int b = 0; b < RB; ++b)
-        if ((r >> b) & 1) o ^= lr[b];
-      return o;
-    };
     for (int o = 0; o < H.nops; ++o) {
       const FOp& op = P[o];
       if (op.kind == F_RELAYOUT) {
         if (fdirty) flush();
+        // two rounds through a half-tile image, split on register bit p
         const uint32_t* x = X + op.coef;
-        for (int b = 0; b < RB; ++b) lr[b] = x[b];
-        for (int b = 0; b < TB; ++b) lt[b] = x[RB + b];
-        std::fill(written.begin(), written.end(), 0);
-        for (int tid = 0; tid < NT; ++tid)
-          for (int r = 0; r < NR; ++r) {
-            const uint32_t off = toff(tid) ^ x[RB + TB] ^ roff(r);
-            if (off % S || off / S >= (uint32_t)(NT * NR)) throw std::runtime_error("lds write out of range");
-            if (written[off / S]++) throw std::runtime_error("lds write collision");
-            lds[off / S] = a[(size_t)tid * NR + r];
-          }
-        for (int b = 0; b < RB; ++b) lr[b] = x[RB + TB + 1 + b];
-        for (int b = 0; b < TB; ++b) lt[b] = x[2 * RB + TB + 1 + b];
-        for (int tid = 0; tid < NT; ++tid)
-          for (int r = 0; r < NR; ++r) {
-            const uint32_t off = toff(tid) ^ roff(r);
-            if (off % S || off / S >= (uint32_t)(NT * NR)) throw std::runtime_error("lds read out of range");
-            a[(size_t)tid * NR + r] = lds[off / S];
-          }
+        const int p = (int)x[0];
+        std::vector<uint32_t> lw(x + 1, x + 1 + RB), tw(x + 1 + RB, x + 1 + RB + TB);
+        const uint32_t xf = x[1 + RB + TB];
+        const uint32_t* y = x + 2 + RB + TB;
+        std::vector<uint32_t> lq(y, y + RB), tq(y + RB, y + RB + TB);
+        const uint32_t half = (uint32_t)(NT * NR / 2);
+        for (int k = 0; k < 2; ++k) {
+          std::fill(written.begin(), written.end(), 0);
+          for (int tid = 0; tid < NT; ++tid)
+            for (int r = 0; r < NR; ++r) {
+              if (((r >> p) & 1) != k) continue;
+              uint32_t off = xf;
+              for (int b = 0; b < TB; ++b) if ((tid >> b) & 1) off ^= tw[b];
+              for (int b = 0; b < RB; ++b) if ((r >> b) & 1) off ^= lw[b];
+              if (off % S || off / S >= half) throw std::runtime_error("lds write out of range");
+              if (written[off / S]++) throw std::runtime_error("lds write collision");
+              lds[off / S] = a[(size_t)tid * NR + r];
+            }
+          for (int tid = 0; tid < NT; ++tid)
+            for (int r = 0; r < NR; ++r) {
+              if (((r >> p) & 1) != k) continue;
+              uint32_t off = 0;
+              for (int b = 0; b < TB; ++b) if ((tid >> b) & 1) off ^= tq[b];
+              for (int b = 0; b < RB; ++b) if ((r >> b) & 1) off ^= lq[b];
+              if (off % S || off / S >= half) throw std::runtime_error("lds read out of range");
+              if (!written[off / S]) throw std::runtime_error("lds read of an unwritten slot");
+              a[(size_t)tid * NR + r] = lds[off / S];
+            }
+        }
         continue;
       }
       const int c0 = op.coef;
