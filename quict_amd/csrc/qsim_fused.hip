@@ -81,57 +81,76 @@ template <typename R> __device__ __forceinline__ cx<R> ld_lds(const char* lds, u
   return ld(reinterpret_cast<const cx<R>*>(lds + off));
 }
 
-template <typename R, int NR, int P>
+// Every op below writes all the registers it may touch without branching on
+// data-dependent conditions: controls select coefficients (the identity where a
+// control fails) or select between old and new values.  Branches per register
+// or lane-predicated regions make the compiler keep a second copy of the whole
+// register tile and move it back after every op.
+//
+// CM (control mode): 0 no controls, 1 register-bit controls only (uniform per
+// register pair), 2 thread-bit controls only (per lane), 3 both.
+
+template <typename R> __device__ __forceinline__ cx<R> sel(bool c, cx<R> x, cx<R> y) {
+  return cx<R>{c ? x.re : y.re, c ? x.im : y.im};
+}
+template <typename R> __device__ __forceinline__ cx<R> one() { return mk<R>((R)1, (R)0); }
+template <typename R> __device__ __forceinline__ cx<R> zero() { return mk<R>((R)0, (R)0); }
+
+template <typename R, int NR, int P, int CM>
 __device__ __forceinline__ void do_dense1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok) {
-  if (!tok) return;  // lanes whose thread-bit controls are not all satisfied
-  const cx<R> m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
+  cx<R> m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
+  if constexpr (CM >= 2) {  // lanes whose thread-bit controls fail apply the identity
+    m00 = sel(tok, m00, one<R>());
+    m01 = sel(tok, m01, zero<R>());
+    m10 = sel(tok, m10, zero<R>());
+    m11 = sel(tok, m11, one<R>());
+  }
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (r & (1 << P)) continue;
-    if (((uint32_t)r & creg) != cregv) continue;
+    cx<R> n00 = m00, n01 = m01, n10 = m10, n11 = m11;
+    if constexpr (CM == 1 || CM == 3) {
+      const bool on = ((uint32_t)r & creg) == cregv;
+      n00 = sel(on, n00, one<R>());
+      n01 = sel(on, n01, zero<R>());
+      n10 = sel(on, n10, zero<R>());
+      n11 = sel(on, n11, one<R>());
+    }
     const int r1 = r | (1 << P);
     const cx<R> x0 = a[r], x1 = a[r1];
-    a[r] = cfma(mul(m00, x0), m01, x1);
-    a[r1] = cfma(mul(m10, x0), m11, x1);
+    a[r] = cfma(mul(n00, x0), n01, x1);
+    a[r1] = cfma(mul(n10, x0), n11, x1);
   }
 }
 
-template <typename R, int NR, int P>
+// a0' = m0 a1, a1' = m1 a0 (pure: plain exchange) where the controls hold.
+template <typename R, int NR, int P, int CM>
 __device__ __forceinline__ void do_perm1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok,
                                          int pure) {
-  if (!tok) return;
-  if (pure) {
+  const cx<R> m0 = m[0], m1 = m[1];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      if (r & (1 << P)) continue;
-      if (((uint32_t)r & creg) != cregv) continue;
-      const int r1 = r | (1 << P);
-      const cx<R> t = a[r];
-      a[r] = a[r1];
-      a[r1] = t;
-    }
-  } else {
-    const cx<R> m0 = m[0], m1 = m[1];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      if (r & (1 << P)) continue;
-      if (((uint32_t)r & creg) != cregv) continue;
-      const int r1 = r | (1 << P);
-      const cx<R> x0 = a[r], x1 = a[r1];
-      a[r] = mul(m0, x1);
-      a[r1] = mul(m1, x0);
-    }
+  for (int r = 0; r < NR; ++r) {
+    if (r & (1 << P)) continue;
+    bool on = true;
+    if constexpr (CM == 1 || CM == 3) on = ((uint32_t)r & creg) == cregv;
+    if constexpr (CM >= 2) on = on && tok;
+    const int r1 = r | (1 << P);
+    const cx<R> x0 = a[r], x1 = a[r1];
+    const cx<R> y0 = pure ? x1 : mul(m0, x1), y1 = pure ? x0 : mul(m1, x0);
+    a[r] = sel(on, y0, x0);
+    a[r1] = sel(on, y1, x1);
   }
 }
 
 // positions P0 (MSB) > P1 (LSB)
-template <typename R, int NR, int P0, int P1>
+template <typename R, int NR, int P0, int P1, int CM>
 __device__ __forceinline__ void do_dense2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok) {
-  if (!tok) return;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (r & ((1 << P0) | (1 << P1))) continue;
-    if (((uint32_t)r & creg) != cregv) continue;
+    bool on = true;
+    if constexpr (CM == 1 || CM == 3) on = ((uint32_t)r & creg) == cregv;
+    if constexpr (CM >= 2) on = on && tok;
     const int idx[4] = {r, r | (1 << P1), r | (1 << P0), r | (1 << P0) | (1 << P1)};
     cx<R> x[4];
 #pragma unroll
@@ -142,30 +161,27 @@ __device__ __forceinline__ void do_dense2(cx<R> (&a)[NR], const cx<R>* m, uint32
       cx<R> acc = mul(m[4 * i], x[0]);
 #pragma unroll
       for (int j = 1; j < 4; ++j) acc = cfma(acc, m[4 * i + j], x[j]);
-      a[idx[i]] = acc;
+      a[idx[i]] = (CM == 0) ? acc : sel(on, acc, x[i]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
 
-template <typename R, int NR, int P0, int P1>
+template <typename R, int NR, int P0, int P1, int CM>
 __device__ __forceinline__ void do_swap2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok,
                                          int pure) {
-  if (!tok) return;
   const cx<R> m0 = m[0], m1 = m[1];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     if (r & ((1 << P0) | (1 << P1))) continue;
-    if (((uint32_t)r & creg) != cregv) continue;
+    bool on = true;
+    if constexpr (CM == 1 || CM == 3) on = ((uint32_t)r & creg) == cregv;
+    if constexpr (CM >= 2) on = on && tok;
     const int i01 = r | (1 << P1), i10 = r | (1 << P0);
     const cx<R> x01 = a[i01], x10 = a[i10];
-    if (pure) {
-      a[i01] = x10;
-      a[i10] = x01;
-    } else {
-      a[i01] = mul(m0, x10);
-      a[i10] = mul(m1, x01);
-    }
+    const cx<R> y01 = pure ? x10 : mul(m0, x10), y10 = pure ? x01 : mul(m1, x01);
+    a[i01] = sel(on, y01, x01);
+    a[i10] = sel(on, y10, x10);
   }
 }
 
@@ -246,13 +262,13 @@ __device__ __forceinline__ void flush_lane(cx<R> (&a)[NR], cx<R>& f) {
 // the payload is a table T[sel * NR + r] (sel = up to two thread-index bits, MSB
 // first) and `creg` is the mask of registers whose entry is not 1 for some sel.
 template <typename R, int NR>
-__device__ __forceinline__ void do_diag(cx<R> (&a)[NR], uint32_t creg, uint32_t sel, const cx<R>* T) {
+__device__ __forceinline__ void do_diag(cx<R> (&a)[NR], uint32_t sel, const cx<R>* T) {
   // per-lane loads from row `sel` (the address is lane-dependent, so the 16..64
-  // table entries never occupy scalar registers)
+  // table entries never occupy scalar registers); every register is multiplied
+  // (entries of untouched registers are 1): no per-register branches
   const cx<R>* Tl = T + sel * NR;
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
-    if ((creg >> r) & 1u) a[r] = mul(ld(Tl + r), a[r]);
+  for (int r = 0; r < NR; ++r) a[r] = mul(ld(Tl + r), a[r]);
 }
 
 // Relayout in two rounds through a half-tile LDS image (see the RELAYOUT
@@ -284,122 +300,148 @@ __device__ __forceinline__ void relayout_half(cx<R> (&a)[1 << RB], char* lds, co
   }
 }
 
+// One op of a pass program applied to the register tile.
+template <typename R, int RB, int TB, bool DK>
+__device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdirty, char* lds, const FOpW& e,
+                                       const cx<R>* __restrict__ C, const cx<R>* __restrict__ G,
+                                       const uint32_t* __restrict__ X, uint32_t tid) {
+  constexpr int NR = 1 << RB;
+  const int kind = (int)(e.w[0] & 0xff);
+  const uint32_t opcoef = e.w[3];
+  if (kind == F_RELAYOUT) {
+    if (fdirty) {
+      flush_lane<R, NR>(a, f);
+      fdirty = false;
+    }
+    const uint32_t* x = X + opcoef;
+    const uint32_t pk = x[0];
+    if (pk == 0) relayout_half<R, RB, TB, 0>(a, lds, x, tid);
+    else if (pk == 1) relayout_half<R, RB, TB, 1>(a, lds, x, tid);
+    else if (pk == 2 || RB < 4) relayout_half<R, RB, TB, 2>(a, lds, x, tid);
+    else relayout_half<R, RB, TB, (RB > 3 ? 3 : 0)>(a, lds, x, tid);
+    return;
+  }
+  const int pure = (int)((e.w[0] >> 8) & 0xff);
+  const int ntt = (int)(e.w[0] >> 24);
+  const int pos0 = (int)(e.w[1] & 0xff), pos1 = (int)((e.w[1] >> 8) & 0xff);
+  const int pos2 = (int)((e.w[1] >> 16) & 0xff), pos3 = (int)(e.w[1] >> 24);
+  const uint32_t creg = e.w[4], cregv = e.w[5];
+  const bool tok = (tid & e.w[6]) == e.w[7];
+  // Payloads: diagonal tables are read per lane from the LDS copy (C); the
+  // uniform payloads of dense / exchange ops come from global memory through
+  // scalar loads (G), so control selects on them are scalar and they never
+  // wait behind the tile's vector loads.
+  const cx<R>* m = C + opcoef;
+  const cx<R>* g = G + opcoef;
+  // control mode: bit 0 register-bit controls, bit 1 thread-bit controls
+  const int cm = (e.w[6] ? 2 : 0) | (creg ? 1 : 0);
+  switch (kind) {
+    case F_DENSE1:
+      switch (pos0 * 4 + cm) {
+#define QS_D1(P, M) \
+  case P * 4 + M: \
+    if constexpr (P < RB) do_dense1<R, NR, P, M>(a, g, creg, cregv, tok); \
+    break;
+#define QS_D1P(P) QS_D1(P, 0) QS_D1(P, 1) QS_D1(P, 2) QS_D1(P, 3)
+        QS_D1P(0) QS_D1P(1) QS_D1P(2) QS_D1P(3) QS_D1P(4)
+#undef QS_D1P
+#undef QS_D1
+      }
+      break;
+    case F_PERM1:
+      switch (pos0 * 4 + cm) {
+#define QS_P1(P, M) \
+  case P * 4 + M: \
+    if constexpr (P < RB) do_perm1<R, NR, P, M>(a, g, creg, cregv, tok, pure); \
+    break;
+#define QS_P1P(P) QS_P1(P, 0) QS_P1(P, 1) QS_P1(P, 2) QS_P1(P, 3)
+        QS_P1P(0) QS_P1P(1) QS_P1P(2) QS_P1P(3) QS_P1P(4)
+#undef QS_P1P
+#undef QS_P1
+      }
+      break;
+    case F_DIAG: {
+      const uint32_t sel = diag_sel(ntt, (int)(e.w[2] & 0xff), (int)((e.w[2] >> 8) & 0xff), tid);
+      if (pure) {
+        diag_lane<R>(f, ntt, sel, m);
+        fdirty = true;
+      } else {
+        do_diag<R, NR>(a, sel, m);
+      }
+      break;
+    }
+    case F_DENSE2:
+    case F_SWAP2: {
+      // controlled two-target ops are rare: one instance per pair, control mode 3
+      // (selects) whenever any control is present
+      const bool ctl = cm != 0;
+      const int code = pos0 * 8 + pos1;
+      const bool sw = kind == F_SWAP2;
+      switch (code) {
+#define QS_D2(A, B)                                                                  \
+  case A * 8 + B:                                                                    \
+    if constexpr (A < RB) {                                                          \
+      if (sw) {                                                                      \
+        if (ctl) do_swap2<R, NR, A, B, 3>(a, g, creg, cregv, tok, pure);             \
+        else do_swap2<R, NR, A, B, 0>(a, g, creg, cregv, tok, pure);                 \
+      } else {                                                                       \
+        if (ctl) do_dense2<R, NR, A, B, 3>(a, g, creg, cregv, tok);                  \
+        else do_dense2<R, NR, A, B, 0>(a, g, creg, cregv, tok);                      \
+      }                                                                              \
+    }                                                                                \
+    break;
+        QS_D2(1, 0) QS_D2(2, 0) QS_D2(2, 1) QS_D2(3, 0) QS_D2(3, 1) QS_D2(3, 2)
+        QS_D2(4, 0) QS_D2(4, 1) QS_D2(4, 2) QS_D2(4, 3)
+#undef QS_D2
+      }
+      break;
+    }
+    case F_DENSE3: {
+      if constexpr (DK) {
+        const uint32_t pm = (1u << pos0) | (1u << pos1) | (1u << pos2);
+        switch (pm) {
+#define QS_D3(M) \
+case M:        \
+  if constexpr ((M >> RB) == 0) do_densek<R, NR, 3, M>(a, m, creg, cregv, tok); \
+  break;
+          QS_D3(7u) QS_D3(11u) QS_D3(13u) QS_D3(14u) QS_D3(19u) QS_D3(21u) QS_D3(22u) QS_D3(25u) QS_D3(26u)
+          QS_D3(28u)
+#undef QS_D3
+        }
+      }
+      break;
+    }
+    case F_DENSE4: {
+      if constexpr (DK) {
+        const uint32_t pm = (1u << pos0) | (1u << pos1) | (1u << pos2) | (1u << pos3);
+        switch (pm) {
+#define QS_D4(M) \
+case M:        \
+  if constexpr ((M >> RB) == 0) do_densek<R, NR, 4, M>(a, m, creg, cregv, tok); \
+  break;
+          QS_D4(15u) QS_D4(23u) QS_D4(27u) QS_D4(29u) QS_D4(30u)
+#undef QS_D4
+        }
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
 // Apply a pass's op list to the register tile `a` (load layout on entry, store
 // layout on exit).  DK: include the in-register 3- and 4-target dense ops.
 template <typename R, int RB, int TB, bool DK>
 __device__ __forceinline__ void run_ops(cx<R> (&a)[1 << RB], char* lds, const PassHdr* __restrict__ H,
                                         const FOpW* __restrict__ P, const cx<R>* __restrict__ C,
-                                        const uint32_t* __restrict__ X, uint32_t tid) {
+                                        const cx<R>* __restrict__ G, const uint32_t* __restrict__ X,
+                                        uint32_t tid) {
   constexpr int NR = 1 << RB;
   const int nops = H->nops;
   cx<R> f = mk<R>((R)1, (R)0);  // deferred per-lane diagonal factor
   bool fdirty = false;
-  for (int o = 0; o < nops; ++o) {
-    const FOpW e = P[o];
-    const int kind = (int)(e.w[0] & 0xff);
-    const uint32_t opcoef = e.w[3];
-    if (kind == F_RELAYOUT) {
-      if (fdirty) {
-        flush_lane<R, NR>(a, f);
-        fdirty = false;
-      }
-      const uint32_t* x = X + opcoef;
-      const uint32_t pk = x[0];
-      if (pk == 0) relayout_half<R, RB, TB, 0>(a, lds, x, tid);
-      else if (pk == 1) relayout_half<R, RB, TB, 1>(a, lds, x, tid);
-      else if (pk == 2 || RB < 4) relayout_half<R, RB, TB, 2>(a, lds, x, tid);
-      else relayout_half<R, RB, TB, (RB > 3 ? 3 : 0)>(a, lds, x, tid);
-      continue;
-    }
-    const int pure = (int)((e.w[0] >> 8) & 0xff);
-    const int ntt = (int)(e.w[0] >> 24);
-    const int pos0 = (int)(e.w[1] & 0xff), pos1 = (int)((e.w[1] >> 8) & 0xff);
-    const int pos2 = (int)((e.w[1] >> 16) & 0xff), pos3 = (int)(e.w[1] >> 24);
-    const uint32_t creg = e.w[4], cregv = e.w[5];
-    const bool tok = (tid & e.w[6]) == e.w[7];
-    const cx<R>* m = C + opcoef;
-    switch (kind) {
-      case F_DENSE1:
-        switch (pos0) {
-#define QS_D1(P) \
-  case P:        \
-    if constexpr (P < RB) do_dense1<R, NR, P>(a, m, creg, cregv, tok); \
-    break;
-          QS_D1(0) QS_D1(1) QS_D1(2) QS_D1(3) QS_D1(4)
-#undef QS_D1
-        }
-        break;
-      case F_PERM1:
-        switch (pos0) {
-#define QS_P1(P) \
-  case P:        \
-    if constexpr (P < RB) do_perm1<R, NR, P>(a, m, creg, cregv, tok, pure); \
-    break;
-          QS_P1(0) QS_P1(1) QS_P1(2) QS_P1(3) QS_P1(4)
-#undef QS_P1
-        }
-        break;
-      case F_DIAG: {
-        const uint32_t sel = diag_sel(ntt, (int)(e.w[2] & 0xff), (int)((e.w[2] >> 8) & 0xff), tid);
-        if (pure) {
-          diag_lane<R>(f, ntt, sel, m);
-          fdirty = true;
-        } else {
-          do_diag<R, NR>(a, creg, sel, m);
-        }
-        break;
-      }
-      case F_DENSE2:
-      case F_SWAP2: {
-        const int code = pos0 * 8 + pos1;
-        const bool sw = kind == F_SWAP2;
-        switch (code) {
-#define QS_D2(A, B)                                                         \
-  case A * 8 + B:                                                           \
-    if constexpr (A < RB) {                                                 \
-      if (sw) do_swap2<R, NR, A, B>(a, m, creg, cregv, tok, pure); \
-      else do_dense2<R, NR, A, B>(a, m, creg, cregv, tok);            \
-    }                                                                       \
-    break;
-          QS_D2(1, 0) QS_D2(2, 0) QS_D2(2, 1) QS_D2(3, 0) QS_D2(3, 1) QS_D2(3, 2)
-          QS_D2(4, 0) QS_D2(4, 1) QS_D2(4, 2) QS_D2(4, 3)
-#undef QS_D2
-        }
-        break;
-      }
-      case F_DENSE3: {
-        if constexpr (DK) {
-          const uint32_t pm = (1u << pos0) | (1u << pos1) | (1u << pos2);
-          switch (pm) {
-#define QS_D3(M) \
-  case M:        \
-    if constexpr ((M >> RB) == 0) do_densek<R, NR, 3, M>(a, m, creg, cregv, tok); \
-    break;
-            QS_D3(7u) QS_D3(11u) QS_D3(13u) QS_D3(14u) QS_D3(19u) QS_D3(21u) QS_D3(22u) QS_D3(25u) QS_D3(26u)
-            QS_D3(28u)
-#undef QS_D3
-          }
-        }
-        break;
-      }
-      case F_DENSE4: {
-        if constexpr (DK) {
-          const uint32_t pm = (1u << pos0) | (1u << pos1) | (1u << pos2) | (1u << pos3);
-          switch (pm) {
-#define QS_D4(M) \
-  case M:        \
-    if constexpr ((M >> RB) == 0) do_densek<R, NR, 4, M>(a, m, creg, cregv, tok); \
-    break;
-            QS_D4(15u) QS_D4(23u) QS_D4(27u) QS_D4(29u) QS_D4(30u)
-#undef QS_D4
-          }
-        }
-        break;
-      }
-      default:
-        break;
-    }
-  }
+  for (int o = 0; o < nops; ++o) run_op<R, RB, TB, DK>(a, f, fdirty, lds, P[o], C, G, X, tid);
   if (fdirty) flush_lane<R, NR>(a, f);
 }
 
@@ -484,7 +526,7 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         a[r] = ld_nt(v + (base | g));
       }
     }
-    run_ops<R, RB, TB, DK>(a, lds_raw, H, P, C, aux, tid);
+    run_ops<R, RB, TB, DK>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
     if (H->sync_store) __syncthreads();
     // store: tile base (uniform) XOR per-lane part (with pending flips) XOR register part
     const uint64_t base = insert_zeros(t, H->sorted, K) ^ gout;
@@ -506,7 +548,7 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
 template <typename R, int RB, int TB>
 static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int* passes,
                        hipStream_t s) {
-  const Config c{RB, TB, L};
+  const Config c{RB, TB, L};  // LR = 0: low bits stay on threads at the load and store
   const int K = RB + TB;
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);

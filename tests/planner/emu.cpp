@@ -221,7 +221,8 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
                          int* stats) {
   using namespace qs;
   try {
-    const Config c{RB, TB, L};
+    Config c{RB, TB, L};
+    if (const char* lr = getenv("QS_FUSION_LOWREGS")) c.LR = atoi(lr);
     FusedProgram prog;
     int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);
     if (rc) return rc;

@@ -29,7 +29,8 @@ from test_oracle import ALL_TYPES
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "planner", "_build", "libplanner_emu.so")
 C128 = (4, 8, 16)  # register bits, thread bits, LDS bytes per amplitude
-C64 = (4, 9, 8)
+C64 = (4, 8, 8)
+C64_K13 = (4, 9, 8)
 
 
 @pytest.fixture(scope="module")
@@ -108,7 +109,7 @@ def dense_multi(n=16):
     return c
 
 
-@pytest.mark.parametrize("shape", [C128, C64], ids=["c128-tile", "c64-tile"])
+@pytest.mark.parametrize("shape", [C128, C64, C64_K13], ids=["c128-tile", "c64-tile", "c64-tile-k13"])
 @pytest.mark.parametrize("n,ngates,seed", [(14, 200, 1), (15, 300, 2), (16, 400, 3)])
 def test_all_gate_types(emu, shape, n, ngates, seed):
     out, ref, st = run_both(emu, all_types(n, ngates, seed), n, shape)
@@ -134,6 +135,16 @@ def test_multi_target_dense(emu, shape):
 def test_low_bit_counts(emu, L):
     c = workloads.random_mix_circuit(16, 300, 21, workloads.RANDOM_GATE_PROBS_60_40)
     out, ref, st = run_both(emu, c, 16, C128, L=L)
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("lr", [1, 2])
+def test_low_bits_in_registers_at_load_and_store(emu, lr, monkeypatch):
+    """Planner variant that leaves up to `lr` low bits in registers at the load
+    and store (fewer relayouts, slower tile accesses; not the default)."""
+    monkeypatch.setenv("QS_FUSION_LOWREGS", str(lr))
+    c = workloads.random_mix_circuit(16, 300, 21, workloads.RANDOM_GATE_PROBS_60_40)
+    out, ref, st = run_both(emu, c, 16, C128)
     assert np.max(np.abs(out - ref)) < 1e-12
 
 
