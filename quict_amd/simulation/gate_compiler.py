@@ -26,8 +26,15 @@ from .. import _native as nat
 _NON_UNITARY = (GateType.measure, GateType.reset, GateType.barrier, GateType.perm)
 
 
+_OFFDIAG = {}
+
+
 def _is_diag(m: np.ndarray) -> bool:
-    return bool(np.all(m[~np.eye(m.shape[0], dtype=bool)] == 0))
+    d = m.shape[0]
+    mask = _OFFDIAG.get(d)
+    if mask is None:
+        mask = _OFFDIAG[d] = ~np.eye(d, dtype=bool)
+    return not m[mask].any()
 
 
 def _controlled_split(mat: np.ndarray, nargs: int) -> Tuple[int, np.ndarray]:
@@ -43,17 +50,13 @@ def _controlled_split(mat: np.ndarray, nargs: int) -> Tuple[int, np.ndarray]:
     return 0, mat
 
 
-def compile_gate(gate: BasicGate, n: int, rec: np.ndarray, keep: list, row_offset: int = 0,
-                 conj: bool = False, dagger: bool = False) -> bool:
-    """Fill one QS_GATE_DTYPE record.  Returns False for gates with no effect (id, barrier).
-
-    ``row_offset`` shifts the amplitude bits (used for the row half of a
-    density matrix); ``conj`` conjugates the matrix (column half);
-    ``dagger`` writes the inverse (conjugate transpose).
-    """
+def _translate(gate: BasicGate, n: int, keep: list, row_offset: int, conj: bool, dagger: bool):
+    """(kind, ncontrols, ntargets, bits, payload, big) of one gate, or None for
+    gates with no effect.  ``payload`` is a complex128 vector (inline
+    coefficients) or None; ``big`` the address of a wider matrix kept in ``keep``."""
     gt = gate.type
     if gt in (GateType.id, GateType.barrier):
-        return False
+        return None
     if gt in _NON_UNITARY:
         raise ValueError(f"{gt.name} is not a unitary gate record")
     args = list(gate.cargs) + list(gate.targs)
@@ -66,89 +69,102 @@ def compile_gate(gate: BasicGate, n: int, rec: np.ndarray, keep: list, row_offse
     nargs = len(args)
     mt = gate.matrix_type
     c, t = gate.controls, gate.targets
-    rec["bits"][:] = 0
-    rec["m"][:] = 0
-    rec["big"] = 0
-
-    def put(vals):
-        v = np.asarray(vals, dtype=np.complex128).ravel()
-        rec["m"][: 2 * len(v)] = np.column_stack([v.real, v.imag]).ravel()
 
     if gt == GateType.unitary and c == 0:
         # classify by structure (a Unitary carries no controls of its own)
         cc, blk = _controlled_split(mat, nargs)
         c, t = cc, nargs - cc
         mat_t = blk
-        if _is_diag(mat_t) and t <= 4:
-            mt = MatrixType.diagonal
-        else:
-            mt = MatrixType.normal
+        mt = MatrixType.diagonal if (_is_diag(mat_t) and t <= 4) else MatrixType.normal
         full_diag = np.diag(mat_t)
     else:
         mat_t = mat[-(1 << t):, -(1 << t):]
         full_diag = None
 
-    rec["ncontrols"] = c
-    rec["ntargets"] = t
-    rec["bits"][: nargs] = bits
-
     if mt in (MatrixType.diagonal, MatrixType.diag_diag) and t <= 4:
         d = np.diag(mat)[-(1 << t):] if full_diag is None else full_diag
         if t == 1 and d[0] == 1:
-            rec["kind"] = nat.GATE_PHASE
-            put([d[1]])
-        else:
-            rec["kind"] = nat.GATE_DIAG
-            put(d)
-    elif mt == MatrixType.control:
-        rec["kind"] = nat.GATE_PHASE
-        put([mat[-1, -1]])
-    elif mt == MatrixType.swap and t == 1:
-        rec["kind"] = nat.GATE_XPERM
-        put([mat_t[0, 1], mat_t[1, 0]])
-    elif mt == MatrixType.swap and t == 2:
-        rec["kind"] = nat.GATE_SWAP
-        put([mat_t[1, 2], mat_t[2, 1]])
-    elif mt == MatrixType.reverse and t == 1:
-        rec["kind"] = nat.GATE_XPERM
-        put([mat[-2, -1], mat[-1, -2]])
-    elif mt == MatrixType.reverse:
-        # reverse-classified multi-target unitary: controls are all args but the last
-        rec["ncontrols"] = nargs - 1
-        rec["ntargets"] = 1
-        rec["kind"] = nat.GATE_XPERM
-        put([mat[-2, -1], mat[-1, -2]])
-    else:
-        # dense target block; a diagonal block becomes DIAG / PHASE
-        if _is_diag(mat_t) and t <= 4:
-            d = np.diag(mat_t)
-            if np.all(d[:-1] == 1):
-                rec["kind"] = nat.GATE_PHASE
-                put([d[-1]])
-            else:
-                rec["kind"] = nat.GATE_DIAG
-                put(d)
-        elif t <= 2:
-            rec["kind"] = nat.GATE_DENSE
-            put(mat_t)
-        else:
-            big = np.ascontiguousarray(np.column_stack([mat_t.real.ravel(), mat_t.imag.ravel()]).ravel())
-            keep.append(big)
-            rec["kind"] = nat.GATE_DENSE
-            rec["big"] = big.ctypes.data
+            return nat.GATE_PHASE, c, t, bits, d[1:2], 0
+        return nat.GATE_DIAG, c, t, bits, d, 0
+    if mt == MatrixType.control:
+        return nat.GATE_PHASE, c, t, bits, mat[-1:, -1], 0
+    if mt == MatrixType.swap and t == 1:
+        return nat.GATE_XPERM, c, t, bits, np.array([mat_t[0, 1], mat_t[1, 0]]), 0
+    if mt == MatrixType.swap and t == 2:
+        return nat.GATE_SWAP, c, t, bits, np.array([mat_t[1, 2], mat_t[2, 1]]), 0
+    if mt == MatrixType.reverse:
+        # reverse-classified gate (also a multi-target unitary): controls are all args but the last
+        return nat.GATE_XPERM, nargs - 1, 1, bits, np.array([mat[-2, -1], mat[-1, -2]]), 0
+    # dense target block; a diagonal block becomes DIAG / PHASE
+    if t <= 4 and _is_diag(mat_t):
+        d = np.diag(mat_t)
+        if np.all(d[:-1] == 1):
+            return nat.GATE_PHASE, c, t, bits, d[-1:], 0
+        return nat.GATE_DIAG, c, t, bits, d, 0
+    if t <= 2:
+        return nat.GATE_DENSE, c, t, bits, mat_t.ravel(), 0
+    big = np.ascontiguousarray(mat_t).view(np.float64).ravel().copy()
+    keep.append(big)
+    return nat.GATE_DENSE, c, t, bits, None, big.ctypes.data
+
+
+def compile_gate(gate: BasicGate, n: int, rec: np.ndarray, keep: list, row_offset: int = 0,
+                 conj: bool = False, dagger: bool = False) -> bool:
+    """Fill one QS_GATE_DTYPE record.  Returns False for gates with no effect (id, barrier).
+
+    ``row_offset`` shifts the amplitude bits (used for the row half of a
+    density matrix); ``conj`` conjugates the matrix (column half);
+    ``dagger`` writes the inverse (conjugate transpose).
+    """
+    r = _translate(gate, n, keep, row_offset, conj, dagger)
+    if r is None:
+        return False
+    kind, c, t, bits, payload, big = r
+    rec["kind"] = kind
+    rec["ncontrols"] = c
+    rec["ntargets"] = t
+    rec["bits"][:] = 0
+    rec["bits"][: len(bits)] = bits
+    rec["m"][:] = 0
+    if payload is not None:
+        v = np.ascontiguousarray(payload, dtype=np.complex128).view(np.float64)
+        rec["m"][: len(v)] = v
+    rec["big"] = big
     return True
 
 
 def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: bool = False,
                   dagger: bool = False) -> Tuple[np.ndarray, list]:
-    """Records for a run of unitary gates; ``keep`` holds arrays referenced by pointer."""
-    arr = np.zeros(len(gates), dtype=nat.QS_GATE_DTYPE)
+    """Records for a run of unitary gates; ``keep`` holds arrays referenced by pointer.
+    The fields are gathered in plain arrays and written column-wise."""
     keep: list = []
+    G = len(gates)
+    kind = np.zeros(G, dtype=np.int32)
+    nc = np.zeros(G, dtype=np.int32)
+    nt = np.zeros(G, dtype=np.int32)
+    bits = np.zeros((G, nat.QS_MAX_GATE_BITS), dtype=np.int32)
+    m = np.zeros((G, 32), dtype=np.float64)
+    big = np.zeros(G, dtype=np.uint64)
     k = 0
     for g in gates:
-        if compile_gate(g, n, arr[k], keep, row_offset, conj, dagger):
-            k += 1
-    return arr[:k].copy() if k != len(arr) else arr, keep
+        r = _translate(g, n, keep, row_offset, conj, dagger)
+        if r is None:
+            continue
+        kind[k], nc[k], nt[k] = r[0], r[1], r[2]
+        bits[k, : len(r[3])] = r[3]
+        if r[4] is not None:
+            v = np.ascontiguousarray(r[4], dtype=np.complex128).view(np.float64)
+            m[k, : len(v)] = v
+        big[k] = r[5]
+        k += 1
+    arr = np.zeros(k, dtype=nat.QS_GATE_DTYPE)
+    arr["kind"] = kind[:k]
+    arr["ncontrols"] = nc[:k]
+    arr["ntargets"] = nt[:k]
+    arr["bits"] = bits[:k]
+    arr["m"] = m[:k]
+    arr["big"] = big[:k]
+    return arr, keep
 
 
 def superop_record(kraus: List[np.ndarray], args: List[int], n: int, rec, keep: list):
