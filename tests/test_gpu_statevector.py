@@ -122,6 +122,31 @@ def test_fused_equals_unfused_26q_and_inverse_roundtrip():
     assert np.max(np.abs(back - e0)) < 1e-11
 
 
+@pytest.mark.parametrize("tile_log2,low_bits,prefetch,precision", [
+    (11, 4, 0, "double"), (13, 4, 0, "double"), (12, 2, 0, "double"), (12, 6, 0, "double"),
+    (12, 4, 1, "double"), (13, 4, 0, "single"), (12, 3, 1, "single")])
+def test_fusion_settings_match_unfused(tile_log2, low_bits, prefetch, precision):
+    """Every tile shape / low-bit count / launch variant reachable through
+    qs_set_fusion and qs_set_fusion_launch gives the per-gate result."""
+    say("fusion settings", tile_log2, low_bits, prefetch, precision)
+    import ctypes
+    from quict_amd import _native as nat
+    lib = nat.load_library()
+    ctx = nat.context(0)
+    n = 20
+    c = workloads.random_mix_circuit(n, 400, 31, workloads.RANDOM_GATE_PROBS_60_40)
+    ref = sim(precision, False).run(c).get()
+    try:
+        nat.check(lib.qs_set_fusion(ctx, tile_log2, low_bits))
+        nat.check(lib.qs_set_fusion_launch(ctx, prefetch, 2))
+        out = sim(precision, True).run(c).get()
+    finally:
+        lib.qs_set_fusion(ctx, 0, 0)
+        lib.qs_set_fusion_launch(ctx, 0, 2)
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert np.max(np.abs(out - ref)) < tol
+
+
 def test_relabel_and_flip_passes_are_exact():
     """Uncontrolled X and Swap gates compile to relabelings of the tile layout
     (no arithmetic): the load and store layouts of such passes differ.  Every
