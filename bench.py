@@ -198,6 +198,75 @@ def run_dm14(lib, ctx, reps=3):
     return out
 
 
+def run_unitary14(lib, ctx, reps=3, cpu=True):
+    """14-qubit circuit unitary (2^28 complex128 entries) of the 300-gate mixed
+    circuit: identity, fused gate passes on the column layout, U x, and the
+    in-place transpose to the row-major matrix; each kernel timed with HIP
+    events on the launch stream.  CPU leg: the restated reference algorithm
+    (CircuitMatrix) at 10 qubits."""
+    import numpy as np
+    import torch
+    from quict_amd import _native as nat
+    from quict_amd import workloads
+    from quict_amd.simulation.unitary import build_columns
+    n = 14
+    D = 1 << n
+    c = workloads.dm14_circuit()
+    stream = nat.stream_ptr(0)
+    s = torch.cuda.current_stream()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    W, passes = build_columns(c.gates, n, nat.QS_C128)
+    x = torch.zeros(D, dtype=torch.complex128, device="cuda:0")
+    x[0] = 1
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t_id, t_gates, t_mv, t_tr, t_all = [], [], [], [], []
+    for _ in range(reps):
+        t_id.append(timed(lambda: nat.check(lib.qs_mat_identity(ctx, ctypes.c_void_p(W.data_ptr()), n, nat.QS_C128,
+                                                                stream))))
+        profile_read(lib, ctx, reset=True)
+        lib.qs_profile_enable(ctx, 1)
+        t0 = time.perf_counter()
+        W2, _ = build_columns(c.gates, n, nat.QS_C128)
+        torch.cuda.synchronize()
+        t_all.append((time.perf_counter() - t0) * 1e3)
+        lib.qs_profile_enable(ctx, 0)
+        ms, cnt = profile_read(lib, ctx, reset=True)
+        t_gates.append(ms[0] + ms[1])
+        t_mv.append(timed(lambda: nat.check(lib.qs_mat_vec(ctx, ctypes.c_void_p(W2.data_ptr()), n, nat.QS_C128, 1,
+                                                           ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                                           stream))))
+        t_tr.append(timed(lambda: nat.check(lib.qs_mat_transpose(ctx, ctypes.c_void_p(W2.data_ptr()), n, nat.QS_C128,
+                                                                 stream))))
+        del W2
+    mat_bytes = D * D * 16
+    med = lambda v: float(np.median(v))
+    out = {"qubits": n, "gates": 300, "passes": passes, "matrix_GiB": mat_bytes / 2**30,
+           "identity_ms": med(t_id), "identity_gbs": mat_bytes / (med(t_id) * 1e-3) / 1e9,
+           "gate_passes_ms": med(t_gates), "gbs_per_pass": passes * 2 * mat_bytes / (med(t_gates) * 1e-3) / 1e9,
+           "matvec_ms": med(t_mv), "matvec_gbs": mat_bytes / (med(t_mv) * 1e-3) / 1e9,
+           "transpose_ms": med(t_tr), "transpose_gbs": 2 * mat_bytes / (med(t_tr) * 1e-3) / 1e9,
+           "columns_wall_ms": med(t_all)}
+    del W
+    torch.cuda.empty_cache()
+    if cpu:
+        from oracle import unitary as ou
+        c10 = workloads.random_mix_circuit(10, 300, 5)
+        t0 = time.perf_counter()
+        ou.circuit_matrix(c10.gates, 10)
+        out["cpu_reference_algorithm_10q_ms"] = (time.perf_counter() - t0) * 1e3
+        out["cpu_note"] = "restated CircuitMatrix (numpy) on a 300-gate 10-qubit circuit; 14 qubits is out of its reach"
+    return out
+
+
 def measured_traffic(n, ngates):
     """HBM bytes per tile-pass launch from the committed PMC summary (FETCH_SIZE +
     WRITE_SIZE, copy-calibrated) when this run is the workload it was measured on."""
@@ -303,6 +372,8 @@ def main():
             secondary[f"qnn_batch_{'c128' if prec == 'double' else 'c64'}"] = run_qnn_batch(prec)
         log("secondary: noisy density matrix 14q")
         secondary["dm14_noisy"] = run_dm14(lib, ctx)
+        log("secondary: circuit unitary 14q")
+        secondary["unitary14"] = run_unitary14(lib, ctx, cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

@@ -164,6 +164,26 @@ int qs_dm_prob_zero(qs_ctx* ctx, const void* rho, int n, int dtype, int bit, dou
 /* rho <- P rho P / p: zero rows/cols whose `bit` != outcome, scale the rest. */
 int qs_dm_collapse(qs_ctx* ctx, void* rho, int n, int dtype, int bit, int outcome, double scale, void* stream);
 
+/* ---- circuit matrices (2^n x 2^n complex, n <= 20) ------------------------ */
+/* The circuit unitary is built column-major: start from the identity and apply
+   the circuit's gates with qs_sv_apply_gates on the 2n-bit vector, bits 0..n-1
+   (qubit q = bit n-1-q, as for a state vector).  The buffer then holds
+   W = U^T (row c = U|c>).  Together these replace
+   CircuitMatrix.get_unitary_matrix (QuICT/core/utils/circuit_matrix.py:69-125,
+   reached from Circuit.matrix, QuICT/core/utils/circuit_info.py:177-200) and
+   its MatrixTensorI / MatrixPermutation / tensor / dot expansions
+   (QuICT/ops/linalg/gpu_calculator.py:13-330). */
+/* m <- identity (the empty circuit's matrix, circuit_info.py:187-195). */
+int qs_mat_identity(qs_ctx* ctx, void* m, int n, int dtype, void* stream);
+/* In-place transpose of a square 2^n x 2^n matrix (turns W into U). */
+int qs_mat_transpose(qs_ctx* ctx, void* m, int n, int dtype, void* stream);
+/* y <- M x (trans = 0, M row-major) or y <- M^T x (trans = 1, i.e. U x when M
+   holds W).  x and y are distinct 2^n-amplitude device vectors.  Replaces
+   dot(U, vec) of UnitarySimulator.run (QuICT/simulation/unitary/
+   unitary_simulator.py:100-104, gpu_calculator.py:13-38).  Accumulates in
+   double for both dtypes; deterministic (fixed summation order). */
+int qs_mat_vec(qs_ctx* ctx, const void* m, int n, int dtype, int trans, const void* x, void* y, void* stream);
+
 /* ---- expectation values and gradients (no reference counterpart) --------- */
 /* Observable sum_k coef_k * P_k with P_k a Pauli string given as two bit masks
    (x-mask: bits carrying X or Y, z-mask: bits carrying Z or Y). */

@@ -90,6 +90,9 @@ SIGNATURES = {
     "qs_dm_trace": (_I, [_P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_dm_prob_zero": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D), _P]),
     "qs_dm_collapse": (_I, [_P, _P, _I, _I, _I, _I, _D, _P]),
+    "qs_mat_identity": (_I, [_P, _P, _I, _I, _P]),
+    "qs_mat_transpose": (_I, [_P, _P, _I, _I, _P]),
+    "qs_mat_vec": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
     "qs_sv_apply_pauli_sum": (_I, [_P, _P, _P, _I, _I, _P, _I, _P]),
     "qs_sv_inner": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_adjoint_gradient": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _I, _P]),

@@ -11,7 +11,7 @@ from typing import List, Union
 import numpy as np
 
 from .gate import BasicGate, build_random_gate, GATE_TYPE_TO_CLASS
-from .gate_type import GateType
+from .gate_type import GateType, MatrixType
 from .operator import NoiseGate, Operator, Trigger
 from .qubit import Qureg
 from .composite_gate import CompositeGate
@@ -254,11 +254,22 @@ class Circuit:
                 f.write(s)
         return s
 
-    def matrix(self, device: str = "CPU") -> np.ndarray:
-        """Dense circuit unitary (brute force; small circuits only)."""
+    def matrix(self, device: str = "CPU"):
+        """The circuit's unitary (QuICT/core/utils/circuit_info.py:177-200); gates
+        of matrix type ``special`` are skipped.  device="GPU": built by the HIP
+        library from the identity with fused tile passes and returned as a
+        DeviceArray (``.get()`` -> numpy); device="CPU": host product of the
+        expanded gate matrices (small circuits)."""
+        if device == "GPU":
+            from ..simulation.unitary import circuit_matrix
+            return circuit_matrix(self, "double" if self._precision == np.complex128 else "single")
+        if device != "CPU":
+            raise qerr.ValueError("Circuit.matrix.device must be CPU or GPU, got", device)
         n = self.width()
         u = np.identity(1 << n, dtype=np.complex128)
         for g in self._gates:
+            if isinstance(g, BasicGate) and g.matrix_type == MatrixType.special:
+                continue
             u = matrix_product_to_circuit(g.matrix.astype(np.complex128), g.cargs + g.targs, n) @ u
         return u.astype(self._precision)
 

@@ -6,8 +6,7 @@ per-backend options) and the result dictionary of
 QuICT/simulation/utils/result.py:5-104:
 ``{"id", "shots", "backend_info": {"device", "backend", "options"},
    "data": {"counts", "state_vector", "density_matrix"}}``.
-Backends: ``state_vector`` and ``density_matrix`` (the unitary backend is not
-part of this build).
+Backends: ``unitary``, ``state_vector`` and ``density_matrix``.
 """
 import os
 
@@ -17,6 +16,7 @@ from ..core import Circuit
 from ..core import exception as qerr
 from .state_vector import StateVectorSimulator
 from .density_matrix import DensityMatrixSimulator
+from .unitary import UnitarySimulator
 
 
 class Result:
@@ -79,17 +79,18 @@ class Result:
 
 class Simulator:
     __DEVICE = ["CPU", "GPU"]
-    __BACKEND = ["state_vector", "density_matrix"]
+    __BACKEND = ["unitary", "state_vector", "density_matrix"]
     __PRECISION = ["single", "double"]
     __OPTIONS = {"state_vector": ["gpu_device_id", "matrix_aggregation", "fusion"],
-                 "density_matrix": ["accumulated_mode", "gpu_device_id", "fusion"]}
+                 "density_matrix": ["accumulated_mode", "gpu_device_id", "fusion"],
+                 "unitary": ["gpu_device_id", "fusion"]}
 
     def __init__(self, device: str = "GPU", backend: str = "state_vector", precision: str = "double",
                  circuit_record: bool = False, amplitude_record: bool = False, output_path: str = None, **options):
         if device != "GPU":
             raise qerr.ValueError("Simulator.device must be 'GPU' in this build, got", device)
         if backend not in self.__BACKEND:
-            raise qerr.ValueError("Simulator.backend must be state_vector / density_matrix, got", backend)
+            raise qerr.ValueError("Simulator.backend must be unitary / state_vector / density_matrix, got", backend)
         if precision not in self.__PRECISION:
             raise qerr.ValueError("Simulator.precision must be single / double, got", precision)
         for k in options:
@@ -99,18 +100,26 @@ class Simulator:
         self._options = options
         if backend == "state_vector":
             self._simulator = StateVectorSimulator(device, precision=precision, **options)
+        elif backend == "unitary":
+            self._simulator = UnitarySimulator(device, precision=precision, **options)
         else:
             self._simulator = DensityMatrixSimulator(device, precision=precision, **options)
         self._result = Result(device, backend, precision, circuit_record, amplitude_record, options, output_path)
 
     def run(self, circuit: Circuit, shots: int = 1, state_vector=None, density_matrix=None, noise_model=None,
             use_previous: bool = False) -> dict:
+        if not isinstance(circuit, Circuit) and self._backend != "unitary":
+            raise qerr.SimulatorOptionsUnmatchedError(
+                f"The unitary matrix input only allows in the unitary backend, not {self._backend}.")
         if (density_matrix is not None or noise_model is not None) and self._backend != "density_matrix":
             raise qerr.SimulatorOptionsUnmatchedError("density_matrix / noise_model need the density_matrix backend")
         if state_vector is not None and self._backend == "density_matrix":
             raise qerr.SimulatorOptionsUnmatchedError("state_vector input is not allowed in the density_matrix backend")
-        self._result.record_circuit(circuit)
-        self._result._width = circuit.width()
+        if isinstance(circuit, Circuit):
+            self._result.record_circuit(circuit)
+            self._result._width = circuit.width()
+        else:
+            self._result._width = int(np.shape(circuit)[0]).bit_length() - 1
         if self._backend == "density_matrix":
             amp = self._simulator.run(circuit, density_matrix, noise_model, use_previous)
         else:
