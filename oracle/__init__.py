@@ -17,6 +17,9 @@ transposed operand fails a check):
   density_matrix.*               reconstructed golden DM outer(sv, sv*), literal dense
                                  U rho U^dag algorithm on <= 6 qubits, closed-form
                                  depolarizing / amplitude-damping channels, CPTP checks
+  unitary.*                      brute-force product of expanded gate matrices on random
+                                 1-7 qubit circuits, golden vector U|0>, kron, unitarity,
+                                 circuit * inverse = identity
   gradients.*                    parameter-shift (exact for Pauli rotations), closed form
                                  <Z> = cos(theta) for Ry(theta)|0>, finite differences
   random circuits (core)         parity unpinned against the reference generator byte for
