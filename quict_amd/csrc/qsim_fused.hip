@@ -167,6 +167,29 @@ __device__ __forceinline__ void do_dense2(cx<R> (&a)[NR], const cx<R>* m, uint32
   }
 }
 
+// Diagonal + anti-diagonal two-target op (positions P0 > P1): per group
+// y_s = d_s x_s + o_s x_(s^3), s = (bit P0, bit P1); REAL: real payload.
+template <typename R, int NR, int P0, int P1, bool REAL>
+__device__ __forceinline__ void do_xx2(cx<R> (&a)[NR], const cx<R>* m) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r & ((1 << P0) | (1 << P1))) continue;
+    const int idx[4] = {r, r | (1 << P1), r | (1 << P0), r | (1 << P0) | (1 << P1)};
+    cx<R> x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) x[j] = a[idx[j]];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if constexpr (REAL) {
+        const R d = m[s].re, o = m[4 + s].re;
+        a[idx[s]] = mk<R>(rfma(o, x[s ^ 3].re, d * x[s].re), rfma(o, x[s ^ 3].im, d * x[s].im));
+      } else {
+        a[idx[s]] = cfma(mul(m[s], x[s]), m[4 + s], x[s ^ 3]);
+      }
+    }
+  }
+}
+
 template <typename R, int NR, int P0, int P1, int CM>
 __device__ __forceinline__ void do_swap2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok,
                                          int pure) {
@@ -371,12 +394,13 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
       break;
     }
     case F_DENSE2:
+    case F_XX2:
     case F_SWAP2: {
       // controlled two-target ops are rare: one instance per pair, control mode 3
-      // (selects) whenever any control is present
+      // (selects) whenever any control is present; XX ops are never controlled
       const bool ctl = cm != 0;
       const int code = pos0 * 8 + pos1;
-      const bool sw = kind == F_SWAP2;
+      const bool sw = kind == F_SWAP2, xx = kind == F_XX2;
       switch (code) {
 #define QS_D2(A, B)                                                                  \
   case A * 8 + B:                                                                    \
@@ -384,6 +408,9 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
       if (sw) {                                                                      \
         if (ctl) do_swap2<R, NR, A, B, 3>(a, g, creg, cregv, tok, pure);             \
         else do_swap2<R, NR, A, B, 0>(a, g, creg, cregv, tok, pure);                 \
+      } else if (xx) {                                                               \
+        if (pure) do_xx2<R, NR, A, B, true>(a, g);                                   \
+        else do_xx2<R, NR, A, B, false>(a, g);                                       \
       } else {                                                                       \
         if (ctl) do_dense2<R, NR, A, B, 3>(a, g, creg, cregv, tok);                  \
         else do_dense2<R, NR, A, B, 0>(a, g, creg, cregv, tok);                      \

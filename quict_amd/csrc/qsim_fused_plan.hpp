@@ -22,6 +22,9 @@ enum FKind : int32_t {
   F_SWAP2 = 5,   // a01' = m0*a10, a10' = m1*a01 (pure exchange when pure != 0)
   F_DENSE3 = 6,
   F_DENSE4 = 7,
+  F_XX2 = 8,     // two targets, entries only where row ^ col is 00 or 11 (noise superoperators,
+                 // Rxx/Ryy/FSim): a_s' = d_s a_s + o_s a_(s^3), payload d_0..d_3, o_0..o_3;
+                 // pure != 0: real payload.  Never controlled.
 };
 
 constexpr int FMAX_TB = 10;
@@ -383,6 +386,29 @@ static int build_op(const qs_gate& g, const std::vector<int>& loc, const Layout&
   for (int i = 0; i < t; ++i) op.pos[i] = rp[order[i]];
   op.kind = (t == 1) ? F_DENSE1 : (t == 2) ? F_DENSE2 : (t == 3) ? F_DENSE3 : F_DENSE4;
   ob.coef = Mp;
+  if (t == 2 && c == 0) {
+    // entries only on the diagonal and the anti-diagonal (row ^ col in {0, 3}):
+    // half the coefficients, and real ones (noise superoperators) need no
+    // imaginary products
+    bool xx = true, real = true;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const double re = Mp[2 * (4 * i + j)], im = Mp[2 * (4 * i + j) + 1];
+        if ((i ^ j) == 1 || (i ^ j) == 2) xx = xx && re == 0.0 && im == 0.0;
+        else real = real && im == 0.0;
+      }
+    if (xx) {
+      op.kind = F_XX2;
+      op.pure = real;
+      ob.coef.assign(16, 0.0);
+      for (int s = 0; s < 4; ++s) {
+        ob.coef[2 * s] = Mp[2 * (5 * s)];
+        ob.coef[2 * s + 1] = Mp[2 * (5 * s) + 1];
+        ob.coef[2 * (4 + s)] = Mp[2 * (4 * s + (s ^ 3))];
+        ob.coef[2 * (4 + s) + 1] = Mp[2 * (4 * s + (s ^ 3)) + 1];
+      }
+    }
+  }
   return QS_OK;
 }
 
@@ -966,10 +992,10 @@ static int build_program(const qs_gate* gates, int ngates, int n, const Config& 
 static void print_program_stats(const FusedProgram& prog, int n, int K, int L, int ngates) {
   // op counts by kind; dense1 / exchange by control mode (bit 0 register
   // controls, bit 1 thread controls); diagonals lane-mode vs table
-  int nrel = 0, kinds[8] = {0}, d1cm[4] = {0}, p1cm[4] = {0}, dlane = 0, dtab = 0;
+  int nrel = 0, kinds[9] = {0}, d1cm[4] = {0}, p1cm[4] = {0}, dlane = 0, dtab = 0;
   for (const FOp& o : prog.ops) {
     if (o.kind == F_RELAYOUT) ++nrel;
-    if (o.kind < 8) ++kinds[o.kind];
+    if (o.kind < 9) ++kinds[o.kind];
     const int cm = (o.cthr ? 2 : 0) | (o.creg ? 1 : 0);
     if (o.kind == F_DENSE1) ++d1cm[cm];
     if (o.kind == F_PERM1) ++p1cm[cm];
@@ -979,10 +1005,11 @@ static void print_program_stats(const FusedProgram& prog, int n, int K, int L, i
   for (auto& h : prog.hdrs) lds += h.uses_lds;
   fprintf(stderr,
           "[qsim fusion] n=%d K=%d L=%d gates=%d passes=%zu fused=%zu with_lds=%d ops=%zu relayouts=%d "
-          "dense1=%d perm1=%d diag=%d dense2=%d swap2=%d dense3=%d dense4=%d "
+          "dense1=%d perm1=%d diag=%d dense2=%d xx2=%d swap2=%d dense3=%d dense4=%d "
           "dense1_cm=%d,%d,%d,%d perm1_cm=%d,%d,%d,%d diag_lane=%d diag_table=%d\n",
           n, K, L, ngates, prog.plans.size(), prog.hdrs.size(), lds, prog.ops.size(), nrel, kinds[F_DENSE1],
-          kinds[F_PERM1], kinds[F_DIAG], kinds[F_DENSE2], kinds[F_SWAP2], kinds[F_DENSE3], kinds[F_DENSE4], d1cm[0],
+          kinds[F_PERM1], kinds[F_DIAG], kinds[F_DENSE2], kinds[F_XX2], kinds[F_SWAP2], kinds[F_DENSE3],
+          kinds[F_DENSE4], d1cm[0],
           d1cm[1], d1cm[2], d1cm[3], p1cm[0], p1cm[1], p1cm[2], p1cm[3], dlane, dtab);
 }
 

@@ -152,6 +152,27 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
             }
             break;
           }
+          case F_XX2: {
+            const int P0 = op.pos[0], P1 = op.pos[1];
+            if (P0 <= P1) throw std::runtime_error("xx2 positions not canonical");
+            if (op.creg || op.cthr) throw std::runtime_error("xx2 op with controls");
+            for (int r = 0; r < NR; ++r) {
+              if (r & ((1 << P0) | (1 << P1))) continue;
+              const int idx[4] = {r, r | (1 << P1), r | (1 << P0), r | (1 << P0) | (1 << P1)};
+              zc x[4];
+              for (int j = 0; j < 4; ++j) x[j] = A[idx[j]];
+              for (int s = 0; s < 4; ++s) {
+                zc d = cz(c0 + s), o = cz(c0 + 4 + s);
+                if (op.pure) {
+                  if (d.imag() != 0.0 || o.imag() != 0.0) throw std::runtime_error("xx2 real payload with imaginary part");
+                  d = zc(d.real(), 0.0);
+                  o = zc(o.real(), 0.0);
+                }
+                A[idx[s]] = d * x[s] + o * x[s ^ 3];
+              }
+            }
+            break;
+          }
           case F_DENSE2:
           case F_SWAP2: {
             const int P0 = op.pos[0], P1 = op.pos[1];
@@ -214,9 +235,9 @@ extern "C" const char* emu_last_error() { return qs::g_err.c_str(); }
 // Plan and run `gates` on the complex128 vector `vec` (2^n entries) with tile
 // shape RB register bits x TB thread bits and L low bits; amp_bytes sets the LDS
 // slot offsets (16 for complex128 tiles, 8 for complex64).  vec == NULL: plan
-// only.  stats (16 ints): passes, fused passes, ops, ops per kind (8), largest
+// only.  stats (17 ints): passes, fused passes, ops, ops per kind 0..7, largest
 // pass coefficient count, real dense1, lane diagonals, table diagonals,
-// lane-controlled ops.
+// lane-controlled ops, diagonal + anti-diagonal two-target ops.
 extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, int RB, int TB, int L, int amp_bytes,
                          int* stats) {
   using namespace qs;
@@ -243,8 +264,8 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
                prog.aux.data());
     }
     if (stats) {
-      int kinds[8] = {0};
-      for (const FOp& o : prog.ops) kinds[o.kind & 7]++;
+      int kinds[9] = {0};
+      for (const FOp& o : prog.ops) kinds[o.kind < 9 ? o.kind : 0]++;
       stats[0] = (int)prog.plans.size();
       stats[1] = (int)prog.hdrs.size();
       stats[2] = (int)prog.ops.size();
@@ -269,6 +290,7 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
       stats[13] = lane;
       stats[14] = dtab;
       stats[15] = lanectl;
+      stats[16] = kinds[F_XX2];
       (void)ctl1;
       (void)pperm;
     }

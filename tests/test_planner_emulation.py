@@ -53,7 +53,7 @@ def run_both(lib, c, n, shape, L=4, seed=0):
     ref = v.copy()
     lib.cpu_apply_gates(ref.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 1)
     out = v.copy()
-    st = np.zeros(16, dtype=np.int32)
+    st = np.zeros(17, dtype=np.int32)
     RB, TB, S = shape
     rc = lib.emu_apply(out.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), RB, TB, L, S,
                        st.ctypes.data_as(ctypes.c_void_p))
@@ -161,9 +161,81 @@ def test_from_zero_state_vs_numpy_oracle(emu):
     recs, keep = compile_gates(c.gates, n)
     v = np.zeros(1 << n, dtype=np.complex128)
     v[0] = 1
-    st = np.zeros(16, dtype=np.int32)
+    st = np.zeros(17, dtype=np.int32)
     rc = emu.emu_apply(v.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 4, 8, 4, 16,
                        st.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
     ref = osv.run(all_types(n, 250, 9))
     assert np.max(np.abs(v - ref)) < 1e-12
+
+
+def dm_records(circuit, n, noise_model):
+    """Records of a noisy circuit on the 2n-bit image of rho (row bits n..2n-1):
+    gate on the row bits, its conjugate on the column bits, one superoperator
+    per single-qubit channel factor (the density-matrix simulator's program)."""
+    from quict_amd.core import NoiseGate
+    from quict_amd.simulation.gate_compiler import compile_gate, superop_record
+    noisy = noise_model.transpile(circuit, True)
+    recs, keep = [], []
+
+    def unitary(g):
+        row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+        col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+        if compile_gate(g, n, row, keep, row_offset=n):
+            compile_gate(g, n, col, keep, row_offset=0, conj=True)
+            recs.extend([row, col])
+
+    for g in noisy.gates:
+        if isinstance(g, NoiseGate):
+            f = g.source[1].factors if g.source is not None else None
+            if f and len(f) > 1 and len(f) == len(g.targs) and all(x.qubits == 1 for x in f):
+                unitary(g.source[0])
+                for fx, q in zip(f, g.targs):
+                    rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+                    superop_record(fx.kraus, [q], n, rec, keep)
+                    recs.append(rec)
+                continue
+            rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+            superop_record(g.noise_matrix, list(g.targs), n, rec, keep)
+            recs.append(rec)
+        else:
+            unitary(g)
+    arr = np.zeros(len(recs), dtype=nat.QS_GATE_DTYPE)
+    for i, r in enumerate(recs):
+        arr[i] = r
+    return arr, keep
+
+
+@pytest.mark.parametrize("shape", [C128, C64])
+def test_noise_superoperators_and_xx_ops(emu, shape):
+    """Density-matrix programs (noise superoperators with real payloads) and
+    Rxx / Ryy / FSim gates (complex payloads) use the diagonal + anti-diagonal
+    two-target op; results must still match the reference kernels."""
+    n = 7
+    c = workloads.random_mix_circuit(n, 60, 12)
+    recs, keep = dm_records(c, n, workloads.dm_noise_model())
+    rng = np.random.RandomState(5)
+    v = rng.randn(1 << (2 * n)) + 1j * rng.randn(1 << (2 * n))
+    v /= np.linalg.norm(v)
+    ref = v.copy()
+    emu.cpu_apply_gates(ref.ctypes.data_as(ctypes.c_void_p), 2 * n, nat.gate_array_ptr(recs), len(recs), 1)
+    out = v.copy()
+    st = np.zeros(17, dtype=np.int32)
+    RB, TB, S = shape
+    rc = emu.emu_apply(out.ctypes.data_as(ctypes.c_void_p), 2 * n, nat.gate_array_ptr(recs), len(recs), RB, TB, 4, S,
+                       st.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0, emu.emu_last_error()
+    assert np.max(np.abs(out - ref)) < 1e-12
+    assert st[16] > 0  # diagonal + anti-diagonal ops were used
+
+    from quict_amd.core import Rxx, Ryy
+    c2 = Circuit(14)
+    random.seed(3)
+    for k in range(40):
+        a, b = random.sample(range(14), 2)
+        [Rxx(0.1 * k + 0.2), Ryy(0.3 * k + 0.1), FSim(0.2 * k, 0.7)][k % 3] | c2([a, b])
+        H | c2(random.randrange(14))
+        X | c2(random.randrange(14))
+    out2, ref2, st2 = run_both(emu, c2, 14, shape, seed=2)
+    assert np.max(np.abs(out2 - ref2)) < 1e-12
+    assert st2[16] > 0
