@@ -171,10 +171,10 @@ def superop_record(kraus: List[np.ndarray], args: List[int], n: int, rec, keep: 
     """Record for a channel sum_k K (.) K^dagger on density-matrix qubits ``args``:
     dense superoperator sum_k K (x) conj(K) on [row bits..., column bits...]."""
     m = len(args)
-    S = np.zeros((1 << (2 * m), 1 << (2 * m)), dtype=np.complex128)
-    for K in kraus:
-        K = np.asarray(K, dtype=np.complex128)
-        S += np.kron(K, K.conj())
+    d = 1 << m
+    Ks = np.asarray(kraus, dtype=np.complex128).reshape(-1, d, d)
+    # sum_k kron(K_k, conj(K_k)): S[(a, c), (b, e)] = sum_k K_k[a, b] conj(K_k[c, e])
+    S = np.einsum("kab,kce->acbe", Ks, Ks.conj()).reshape(d * d, d * d)
     row_bits = [2 * n - 1 - a for a in args]
     col_bits = [n - 1 - a for a in args]
     rec["kind"] = nat.GATE_DENSE
@@ -184,10 +184,10 @@ def superop_record(kraus: List[np.ndarray], args: List[int], n: int, rec, keep: 
     rec["bits"][: 2 * m] = row_bits + col_bits
     rec["m"][:] = 0
     rec["big"] = 0
+    v = np.ascontiguousarray(S).view(np.float64).ravel()
     if 2 * m <= 2:
-        v = S.ravel()
-        rec["m"][: 2 * len(v)] = np.column_stack([v.real, v.imag]).ravel()
+        rec["m"][: len(v)] = v
     else:
-        big = np.ascontiguousarray(np.column_stack([S.real.ravel(), S.imag.ravel()]).ravel())
+        big = v.copy()
         keep.append(big)
         rec["big"] = big.ctypes.data
