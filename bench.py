@@ -13,6 +13,7 @@ cpu_baseline = the C restatement of the reference CPU simulator (one pass per
 gate, gather / mat-vec / scatter) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--qubits 30] [--no-secondary]
+       python bench.py --only {dm14,unitary14,adjoint20,qnn_c128,qnn_c64}   (one secondary, for profiling)
 """
 import argparse
 import ctypes
@@ -307,6 +308,8 @@ def main():
     ap.add_argument("--gates", type=int, default=2000)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["dm14", "unitary14", "adjoint20", "qnn_c128", "qnn_c64"],
+                    help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the simulator state lives on a single MI355X; "
@@ -318,6 +321,15 @@ def main():
 
     lib = nat.load_library()
     ctx = nat.context(0)
+    if args.only:
+        log(f"secondary workload {args.only} alone")
+        only = {"dm14": lambda: run_dm14(lib, ctx),
+                "unitary14": lambda: run_unitary14(lib, ctx, cpu=not args.no_cpu),
+                "adjoint20": run_adjoint_20q,
+                "qnn_c128": lambda: run_qnn_batch("double"),
+                "qnn_c64": lambda: run_qnn_batch("single")}[args.only]()
+        print(json.dumps({"workload": args.only, **only}))
+        return
     n = args.qubits
     log(f"building circuit n={n} gates={args.gates}")
     circuit = workloads.sv30_circuit(n=n, ngates=args.gates)
