@@ -589,13 +589,21 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
   std::vector<int> remaining(ngates);
   for (int i = 0; i < ngates; ++i) remaining[i] = i;
   const uint64_t low = (1ull << c.L) - 1;
-  std::vector<uint64_t> gm(ngates);
+  // gm: every bit of the gate; gx: the bits it mixes (non-diagonal targets; all
+  // bits of a gate the tile kernel cannot take).  Two gates commute when every
+  // bit they share is used diagonally by both.
+  std::vector<uint64_t> gm(ngates), gx(ngates);
   std::vector<int> gcost(ngates);
   std::vector<char> ok(ngates);
   for (int i = 0; i < ngates; ++i) {
-    gm[i] = gate_mask(gates[i]);
-    gcost[i] = coef_cost(gates[i], 1 << c.RB);
-    ok[i] = fusable(gates[i], c);
+    const qs_gate& g = gates[i];
+    gm[i] = gate_mask(g);
+    gcost[i] = coef_cost(g, 1 << c.RB);
+    ok[i] = fusable(g, c);
+    gx[i] = 0;
+    if (!ok[i]) gx[i] = gm[i];
+    else if (!gate_diag_like(g))
+      for (int k = 0; k < g.ntargets; ++k) gx[i] |= 1ull << g.bits[g.ncontrols + k];
   }
   std::vector<char> taken(ngates, 0);
   while (!remaining.empty()) {
@@ -613,12 +621,15 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     uint64_t S = low;
     int coef_budget = FMAX_PASS_COEF;  // LDS-staged payload per pass (complex entries)
     for (;;) {
-      uint64_t blocked = 0;
+      // a later gate may move ahead of the gates still waiting only if it
+      // commutes with all of them
+      uint64_t wait_all = 0, wait_mix = 0;
       int best = -1, best_new = 1 << 30;
       for (int gi : remaining) {
         if (taken[gi]) continue;
         const uint64_t m = gm[gi];
-        if (!(m & blocked) && ok[gi] && gcost[gi] <= coef_budget && __builtin_popcountll(S | m) <= K) {
+        const bool blocked = (m & wait_mix) || (gx[gi] & wait_all);
+        if (!blocked && ok[gi] && gcost[gi] <= coef_budget && __builtin_popcountll(S | m) <= K) {
           const int nb = __builtin_popcountll(m & ~S);
           if (nb < best_new) {
             best_new = nb;
@@ -626,7 +637,8 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
             if (nb == 0) break;
           }
         }
-        blocked |= m;  // later gates sharing a bit with a waiting gate must stay behind it
+        wait_all |= m;
+        wait_mix |= gx[gi];
       }
       if (best < 0) break;
       taken[best] = 1;
