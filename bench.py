@@ -94,7 +94,7 @@ def run_stream_copy(n, lib, ctx, reps=5):
     return out
 
 
-def run_adjoint_20q(reps=3):
+def run_adjoint_20q(reps=3, cpu=False):
     """20-qubit 12-layer hardware-efficient ansatz, 480 parameters, H = sum ZZ + 0.5 sum X:
     adjoint gradient vs the parameter-shift gradient, both on this GPU."""
     import numpy as np
@@ -114,10 +114,27 @@ def run_adjoint_20q(reps=3):
     ps = parameter_shift_gradient(pc, th, H)
     t_ps = time.perf_counter() - t0
     err = float(np.max(np.abs(g - ps)))
-    return {"qubits": n, "params": pc.nparams, "energy": E, "adjoint_ms": t_adj * 1e3,
-            "adjoint_grads_per_s": pc.nparams / t_adj, "shift_ms": t_ps * 1e3,
-            "shift_grads_per_s": pc.nparams / t_ps, "speedup_vs_shift": t_ps / t_adj,
-            "max_abs_err_vs_shift": err}
+    out = {"qubits": n, "params": pc.nparams, "energy": E, "adjoint_ms": t_adj * 1e3,
+           "adjoint_grads_per_s": pc.nparams / t_adj, "shift_ms": t_ps * 1e3,
+           "shift_grads_per_s": pc.nparams / t_ps, "speedup_vs_shift": t_ps / t_adj,
+           "max_abs_err_vs_shift": err}
+    if cpu:
+        # CPU parameter shift: two expectations per parameter, each the whole
+        # circuit through the C restatement (all cores) plus <psi|H|psi>
+        from quict_amd.simulation.gate_compiler import compile_gates
+        from oracle import cpu_baseline, gradients as og
+        threads = os.cpu_count() or 1
+        recs, keep = compile_gates(pc.bind(th).gates, n)
+        v = np.zeros(1 << n, dtype=np.complex128)
+        v[0] = 1
+        t0 = time.perf_counter()
+        cpu_baseline.apply_gates(v, n, recs, threads)
+        e_cpu = float(np.real(np.vdot(v, og.pauli_apply(v, n, workloads.zz_x_hamiltonian(n)))))
+        t_e = time.perf_counter() - t0
+        out["cpu_baseline"] = {"shift_grads_per_s": 1.0 / (2 * t_e), "expectation_s": t_e, "threads": threads,
+                               "kind": "port", "energy_check": abs(e_cpu - E),
+                               "sample": "one expectation (708 gates + H) on the C restatement; shift = 2 per parameter"}
+    return out
 
 
 def run_qnn_batch(precision, reps=3, sample_circuits=3, sample_params=6):
@@ -167,7 +184,7 @@ def run_qnn_batch(precision, reps=3, sample_circuits=3, sample_params=6):
                     "2(E - t) * shift derivative"}
 
 
-def run_dm14(lib, ctx, reps=3):
+def run_dm14(lib, ctx, reps=3, cpu=False):
     """14-qubit noisy density matrix (2^28 complex128 entries): 300-gate circuit with
     depolarizing / amplitude-damping channels, then diagonal and trace."""
     import torch
@@ -196,7 +213,47 @@ def run_dm14(lib, ctx, reps=3):
            "trace": [tr.real, tr.imag], "diag_sum": float(d.sum())}
     del sim
     torch.cuda.empty_cache()
+    if cpu:
+        out["cpu_baseline"] = cpu_dm14(c, nm)
     return out
+
+
+def cpu_dm14(c, nm, sample_records=6, dense_sizes=(5, 6, 7, 8)):
+    """CPU legs of the noisy density matrix: (a) the restated superoperator path
+    (C restatement, one pass per record over the 2n-bit image) at 14 qubits, a
+    sample of records on all cores scaled to the whole program; (b) the literal
+    reference algorithm (dense U rho U^dagger per gate, Kraus sums) timed at
+    5..8 qubits and extrapolated to 14 along its fitted exponent."""
+    import numpy as np
+    from quict_amd import workloads
+    from quict_amd.simulation.density_matrix import program_records
+    from oracle import cpu_baseline, density_matrix as odm
+    n = 14
+    recs, keep = program_records(nm.transpile(c, True).gates, n)
+    rho = np.zeros(1 << (2 * n), dtype=np.complex128)
+    rho[0] = 1
+    threads = os.cpu_count() or 1
+    k = min(sample_records, len(recs))
+    t0 = time.perf_counter()
+    cpu_baseline.apply_gates(rho, 2 * n, recs[:k], threads)
+    t_rec = (time.perf_counter() - t0) / k
+    del rho
+    dense = {}
+    for m in dense_sizes:
+        cm = workloads.random_mix_circuit(m, 300, 5)
+        noisy = nm.transpile(cm, True)
+        t0 = time.perf_counter()
+        odm.run_dense(noisy)
+        dense[m] = time.perf_counter() - t0
+    xs = np.array(list(dense.keys()), dtype=float)
+    ys = np.log(np.array(list(dense.values())))
+    slope, icpt = np.polyfit(xs, ys, 1)
+    return {"superop_path_s_14q": t_rec * len(recs), "superop_path_threads": threads,
+            "superop_path_sample": f"{k} of {len(recs)} records timed, scaled",
+            "dense_reference_s": {str(m): v for m, v in dense.items()},
+            "dense_reference_s_14q_extrapolated": float(np.exp(icpt + slope * n)),
+            "dense_reference_growth_per_qubit": float(np.exp(slope)),
+            "kind": "port", "note": "dense figure at 14 qubits is EXTRAPOLATED from 5..8 qubits (numpy, 1 process)"}
 
 
 def run_unitary14(lib, ctx, reps=3, cpu=True):
@@ -283,6 +340,34 @@ def measured_traffic(n, ngates):
     return vals.get("traffic_bytes")
 
 
+def cpu_sv20(c20, shot_sample=4):
+    """CPU legs of the 20-qubit configuration: the 400 gates through the C
+    restatement (one pass per gate) on 1 thread and on all cores, and the
+    reference's per-shot sampling (copy the state, then measure and collapse
+    qubit after qubit) on a few shots, scaled to 1024."""
+    import numpy as np
+    import random
+    from quict_amd.simulation.gate_compiler import compile_gates
+    from oracle import cpu_baseline
+    n = 20
+    recs, keep = compile_gates(c20.gates, n)
+    out = {"kind": "port"}
+    for threads in (1, os.cpu_count() or 1):
+        v = np.zeros(1 << n, dtype=np.complex128)
+        v[0] = 1
+        t0 = time.perf_counter()
+        cpu_baseline.apply_gates(v, n, recs, threads)
+        dt = time.perf_counter() - t0
+        out[f"gates_per_s_{threads}t"] = len(c20.gates) / dt
+    random.seed(1001)
+    u = np.array([random.random() for _ in range(shot_sample * n)], dtype=np.float64)
+    t0 = time.perf_counter()
+    cpu_baseline.sample(v, n, u, shot_sample)
+    out["sample_1024_s"] = (time.perf_counter() - t0) / shot_sample * 1024
+    out["sample"] = f"all 400 gates; {shot_sample} shots timed (1 thread) and scaled to 1024"
+    return out
+
+
 def cpu_baseline_sample(n_cpu=28, ngates=10, threads=1):
     import numpy as np
     from quict_amd import workloads
@@ -323,9 +408,9 @@ def main():
     ctx = nat.context(0)
     if args.only:
         log(f"secondary workload {args.only} alone")
-        only = {"dm14": lambda: run_dm14(lib, ctx),
+        only = {"dm14": lambda: run_dm14(lib, ctx, cpu=not args.no_cpu),
                 "unitary14": lambda: run_unitary14(lib, ctx, cpu=not args.no_cpu),
-                "adjoint20": run_adjoint_20q,
+                "adjoint20": lambda: run_adjoint_20q(cpu=not args.no_cpu),
                 "qnn_c128": lambda: run_qnn_batch("double"),
                 "qnn_c64": lambda: run_qnn_batch("single")}[args.only]()
         print(json.dumps({"workload": args.only, **only}))
@@ -377,13 +462,16 @@ def main():
         del s20
         secondary["sv20_400"] = {"ms_per_step": dt20 / 5 * 1e3, "gates_per_s": 400 / (dt20 / 5), "passes": p20,
                                  "sample_1024_ms": t_shots * 1e3}
+        if not args.no_cpu:
+            log("secondary: 20-qubit circuit and shots on the CPU restatement")
+            secondary["sv20_400"]["cpu_baseline"] = cpu_sv20(c20)
         log("secondary: adjoint gradient 20q x 12 layers vs parameter shift")
-        secondary["adjoint_20q"] = run_adjoint_20q()
+        secondary["adjoint_20q"] = run_adjoint_20q(cpu=not args.no_cpu)
         for prec in ("double", "single"):
             log(f"secondary: batched QNN {prec} (autotune + timed)")
             secondary[f"qnn_batch_{'c128' if prec == 'double' else 'c64'}"] = run_qnn_batch(prec)
         log("secondary: noisy density matrix 14q")
-        secondary["dm14_noisy"] = run_dm14(lib, ctx)
+        secondary["dm14_noisy"] = run_dm14(lib, ctx, cpu=not args.no_cpu)
         log("secondary: circuit unitary 14q")
         secondary["unitary14"] = run_unitary14(lib, ctx, cpu=not args.no_cpu)
 

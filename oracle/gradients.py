@@ -25,8 +25,10 @@ def pauli_apply(vec: np.ndarray, n: int, terms) -> np.ndarray:
             if ch == "Y":
                 ny += 1
         src = idx ^ x
-        sign = np.array([(-1) ** bin(int(i) & z).count("1") for i in src]) if n <= 12 else \
-            (-1.0) ** np.array([bin(int(i) & z).count("1") for i in src])
+        par = src & z  # parity of the Z/Y bits of the source index
+        for sh in (32, 16, 8, 4, 2, 1):
+            par ^= par >> sh
+        sign = 1.0 - 2.0 * (par & 1)
         out += c * (1j ** ny) * sign * vec[src]
     return out
 

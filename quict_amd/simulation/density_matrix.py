@@ -31,6 +31,51 @@ from .device_array import DeviceArray
 from .gate_compiler import compile_gate, superop_record
 
 
+def op_records(g, n: int, keep: list) -> list:
+    """Records of one unitary gate or noise channel on the 2n-bit image of rho
+    (row bits n..2n-1, column bits 0..n-1): a gate U is U on its row bits and
+    conj(U) on its column bits; a channel is its superoperator.  A tensor-product
+    channel applied after a gate (accumulated mode) is the gate's two records
+    followed by one superoperator per single-qubit factor."""
+    out = []
+
+    def unitary(gate):
+        row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+        col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+        if compile_gate(gate, n, row, keep, row_offset=n):
+            compile_gate(gate, n, col, keep, row_offset=0, conj=True)
+            out.extend([row, col])
+
+    if isinstance(g, NoiseGate):
+        factors = g.source[1].factors if g.source is not None else None
+        if factors and len(factors) > 1 and len(factors) == len(g.targs) and all(f.qubits == 1 for f in factors):
+            unitary(g.source[0])
+            for f, q in zip(factors, g.targs):
+                rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+                superop_record(f.kraus, [q], n, rec, keep)
+                out.append(rec)
+        else:
+            rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
+            superop_record(g.noise_matrix, list(g.targs), n, rec, keep)
+            out.append(rec)
+    else:
+        unitary(g)
+    return out
+
+
+def program_records(ops, n: int):
+    """Record array of a measurement-free list of gates and channels (the program
+    the density-matrix simulator hands to qs_sv_apply_gates on 2n bits)."""
+    keep: list = []
+    recs = []
+    for g in ops:
+        recs.extend(op_records(g, n, keep))
+    arr = np.zeros(len(recs), dtype=nat.QS_GATE_DTYPE)
+    for i, r in enumerate(recs):
+        arr[i] = r
+    return arr, keep
+
+
 class DensityMatrixSimulator:
     def __init__(self, device: str = "GPU", precision: str = "double", accumulated_mode: bool = False,
                  gpu_device_id: int = 0, fusion: bool = True):
@@ -125,36 +170,14 @@ class DensityMatrixSimulator:
             self.last_passes += passes.value
             recs.clear()
 
-        def unitary(g):
-            row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-            col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-            if compile_gate(g, n, row, keep, row_offset=n):
-                compile_gate(g, n, col, keep, row_offset=0, conj=True)
-                recs.append(row)
-                recs.append(col)
-
         for g in circuit.gates:
-            if isinstance(g, NoiseGate):
-                factors = g.source[1].factors if g.source is not None else None
-                if factors and len(factors) > 1 and len(factors) == len(g.targs) and \
-                        all(f.qubits == 1 for f in factors):
-                    # K = (A (x) B ...) U: apply U, then each single-qubit factor channel
-                    unitary(g.source[0])
-                    for f, q in zip(factors, g.targs):
-                        rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-                        superop_record(f.kraus, [q], n, rec, keep)
-                        recs.append(rec)
-                    continue
-                rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-                superop_record(g.noise_matrix, list(g.targs), n, rec, keep)
-                recs.append(rec)
-            elif isinstance(g, BasicGate) and g.type == GateType.measure:
+            if isinstance(g, BasicGate) and g.type == GateType.measure:
                 flush()
                 self.apply_measure(g.targ)
             elif isinstance(g, BasicGate) and g.type in (GateType.reset, GateType.perm):
                 raise qerr.GateTypeNotImplementError(f"{g.type.name} in the density-matrix simulator")
-            elif isinstance(g, BasicGate):
-                unitary(g)
+            elif isinstance(g, (BasicGate, NoiseGate)):
+                recs.extend(op_records(g, n, keep))
             else:
                 raise qerr.TypeError("DensityMatrixSimulator takes BasicGate / NoiseGate, got", type(g))
         flush()

@@ -170,40 +170,9 @@ def test_from_zero_state_vs_numpy_oracle(emu):
 
 
 def dm_records(circuit, n, noise_model):
-    """Records of a noisy circuit on the 2n-bit image of rho (row bits n..2n-1):
-    gate on the row bits, its conjugate on the column bits, one superoperator
-    per single-qubit channel factor (the density-matrix simulator's program)."""
-    from quict_amd.core import NoiseGate
-    from quict_amd.simulation.gate_compiler import compile_gate, superop_record
-    noisy = noise_model.transpile(circuit, True)
-    recs, keep = [], []
-
-    def unitary(g):
-        row = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-        col = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-        if compile_gate(g, n, row, keep, row_offset=n):
-            compile_gate(g, n, col, keep, row_offset=0, conj=True)
-            recs.extend([row, col])
-
-    for g in noisy.gates:
-        if isinstance(g, NoiseGate):
-            f = g.source[1].factors if g.source is not None else None
-            if f and len(f) > 1 and len(f) == len(g.targs) and all(x.qubits == 1 for x in f):
-                unitary(g.source[0])
-                for fx, q in zip(f, g.targs):
-                    rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-                    superop_record(fx.kraus, [q], n, rec, keep)
-                    recs.append(rec)
-                continue
-            rec = np.zeros(1, dtype=nat.QS_GATE_DTYPE)[0]
-            superop_record(g.noise_matrix, list(g.targs), n, rec, keep)
-            recs.append(rec)
-        else:
-            unitary(g)
-    arr = np.zeros(len(recs), dtype=nat.QS_GATE_DTYPE)
-    for i, r in enumerate(recs):
-        arr[i] = r
-    return arr, keep
+    """The density-matrix simulator's record program for a noisy circuit."""
+    from quict_amd.simulation.density_matrix import program_records
+    return program_records(noise_model.transpile(circuit, True).gates, n)
 
 
 @pytest.mark.parametrize("shape", [C128, C64])
