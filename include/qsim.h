@@ -116,6 +116,26 @@ int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
    current one; prefetch == 0 launches one workgroup per tile. */
 int qs_set_fusion_launch(qs_ctx* ctx, int prefetch, int wg_per_cu);
 
+/* Captured gate sequences (HIP graphs).  qs_graph_capture plans the sequence
+   once (the same fused passes as qs_sv_apply_gates with fuse=1), uploads the
+   program into memory the graph owns and captures the passes on `vec` in a HIP
+   graph; init_zero != 0 puts a |0...0> initialisation in front.  Every
+   qs_graph_launch then replays them on `stream` with no host-side planning:
+   repeated runs of one circuit (sampling loops, serving) cost only the GPU
+   passes.  The graph is bound to `vec` (same buffer, size and dtype) until
+   qs_graph_destroy.  Sequences with a gate the tile kernel cannot take (dense
+   on more than 4 targets) and registers smaller than 2^(tile+1) amplitudes are
+   not captured (QS_ERR_UNSUPPORTED).  Replaces the per-run gate loop of
+   StateVectorSimulator.run (statevector_simulator.py:131-166) for a fixed
+   circuit; the reference has no captured form. */
+typedef struct qs_graph qs_graph;
+int qs_graph_capture(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int init_zero,
+                     qs_graph** out);
+int qs_graph_launch(qs_graph* g, void* stream);
+/* Full-state passes one launch executes (-1 for NULL). */
+int qs_graph_passes(const qs_graph* g);
+void qs_graph_destroy(qs_graph* g);
+
 /* P(bit == 0) = sum |a|^2 over the half with that bit clear, reduced in double.
    Replaces measured_prob_calculate (gate_function.py:1751-1776). */
 int qs_sv_prob_zero(qs_ctx* ctx, const void* vec, int n, int dtype, int bit, double* p0, void* stream);

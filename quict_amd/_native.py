@@ -90,6 +90,10 @@ SIGNATURES = {
     "qs_dm_trace": (_I, [_P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_dm_prob_zero": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D), _P]),
     "qs_dm_collapse": (_I, [_P, _P, _I, _I, _I, _I, _D, _P]),
+    "qs_graph_capture": (_I, [_P, _P, _I, _I, _P, _I, _I, ctypes.POINTER(_P)]),
+    "qs_graph_launch": (_I, [_P, _P]),
+    "qs_graph_passes": (_I, [_P]),
+    "qs_graph_destroy": (None, [_P]),
     "qs_mat_identity": (_I, [_P, _P, _I, _I, _P]),
     "qs_mat_transpose": (_I, [_P, _P, _I, _I, _P]),
     "qs_mat_vec": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),

@@ -572,51 +572,51 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
   }
 }
 
-template <typename R, int RB, int TB>
-static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int* passes,
-                       hipStream_t s) {
-  const Config c{RB, TB, L};  // LR = 0: low bits stay on threads at the load and store
-  const int K = RB + TB;
-  FusedProgram prog;
-  int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
-  if (rc) return rc;
-  if (const char* dbg = getenv("QS_FUSION_DEBUG"))
-    if (dbg[0] == '1') print_program_stats(prog, n, K, L, ngates);
-  // upload program: [hdrs][ops][aux][coef in R]
+// Device image of a pass program: [hdrs][ops as FOpW][aux][coef in R], each
+// part 256-byte aligned.
+struct ProgImage {
+  std::vector<char> host;
+  uint64_t hb = 0, ob = 0, xb = 0, cb = 0;
+};
+
+template <typename R> static void pack_program(const FusedProgram& prog, ProgImage& im) {
   auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
-  const uint64_t hb = al(prog.hdrs.size() * sizeof(PassHdr));
-  const uint64_t ob = al(prog.ops.size() * sizeof(FOpW));
-  const uint64_t xb = al(prog.aux.size() * sizeof(uint32_t));
+  im.hb = al(prog.hdrs.size() * sizeof(PassHdr));
+  im.ob = al(prog.ops.size() * sizeof(FOpW));
+  im.xb = al(prog.aux.size() * sizeof(uint32_t));
   const uint64_t ncoef = prog.coef.size() / 2;
-  const uint64_t cb = al(ncoef * sizeof(cx<R>));
-  rc = ensure_prog(ctx, hb + ob + xb + cb + 256);
-  if (rc) return rc;
-  std::vector<char> host(hb + ob + xb + cb, 0);
-  if (!prog.hdrs.empty()) std::memcpy(host.data(), prog.hdrs.data(), prog.hdrs.size() * sizeof(PassHdr));
+  im.cb = al(ncoef * sizeof(cx<R>));
+  im.host.assign(im.hb + im.ob + im.xb + im.cb, 0);
+  char* h = im.host.data();
+  if (!prog.hdrs.empty()) std::memcpy(h, prog.hdrs.data(), prog.hdrs.size() * sizeof(PassHdr));
   for (size_t i = 0; i < prog.ops.size(); ++i) {
     const FOpW e = encode_op(prog.ops[i]);
-    std::memcpy(host.data() + hb + i * sizeof(FOpW), &e, sizeof(FOpW));
+    std::memcpy(h + im.hb + i * sizeof(FOpW), &e, sizeof(FOpW));
   }
-  if (!prog.aux.empty()) std::memcpy(host.data() + hb + ob, prog.aux.data(), prog.aux.size() * sizeof(uint32_t));
-  cx<R>* hc = reinterpret_cast<cx<R>*>(host.data() + hb + ob + xb);
+  if (!prog.aux.empty()) std::memcpy(h + im.hb + im.ob, prog.aux.data(), prog.aux.size() * sizeof(uint32_t));
+  cx<R>* hc = reinterpret_cast<cx<R>*>(h + im.hb + im.ob + im.xb);
   for (uint64_t i = 0; i < ncoef; ++i) hc[i] = mk<R>((R)prog.coef[2 * i], (R)prog.coef[2 * i + 1]);
-  char* dprog = (char*)ctx->prog;
-  if (!host.empty()) QS_CHECK_HIP(hipMemcpyAsync(dprog, host.data(), host.size(), hipMemcpyHostToDevice, s));
+}
+
+// Launch every pass of `prog` (image at dprog) on stream s.  prof: bracket the
+// launches with the context's timing events (never while capturing a graph).
+template <typename R, int RB, int TB>
+static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, const FusedProgram& prog,
+                         const char* dprog, const ProgImage& im, hipStream_t s, bool prof, int* npass_out) {
+  constexpr int K = RB + TB;
   const PassHdr* dH = (const PassHdr*)dprog;
-  const FOpW* dO = (const FOpW*)(dprog + hb);
-  const uint32_t* dX = (const uint32_t*)(dprog + hb + ob);
-  const cx<R>* dC = (const cx<R>*)(dprog + hb + ob + xb);
+  const FOpW* dO = (const FOpW*)(dprog + im.hb);
+  const uint32_t* dX = (const uint32_t*)(dprog + im.hb + im.ob);
+  const cx<R>* dC = (const cx<R>*)(dprog + im.hb + im.ob + im.xb);
   const uint64_t tiles = 1ull << (n - K);
   const size_t lds_bytes = (size_t)(1u << (K - 1)) * sizeof(cx<R>);  // half-tile relayout image
-  int npass = 0;
+  int npass = 0, rc = QS_OK;
   for (size_t i = 0; i < prog.plans.size(); ++i) {
     if (!prog.plans[i].fused) {
-      rc = prof_begin(ctx, PROF_GATE, s);
-      if (rc) return rc;
+      if (prof && (rc = prof_begin(ctx, PROF_GATE, s))) return rc;
       rc = apply_gate(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &gates[prog.plans[i].single], s);
       if (rc) return rc;
-      rc = prof_end(ctx, s);
-      if (rc) return rc;
+      if (prof && (rc = prof_end(ctx, s))) return rc;
       ++npass;
       continue;
     }
@@ -624,8 +624,7 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
     const PassHdr& hd = prog.hdrs[h];
     const size_t lds_relayout = hd.uses_lds ? lds_bytes : 0;
     const size_t lds = hd.has_dk ? lds_relayout : lds_relayout + (size_t)hd.ncoef * sizeof(cx<R>);
-    rc = prof_begin(ctx, PROF_FUSED, s);
-    if (rc) return rc;
+    if (prof && (rc = prof_begin(ctx, PROF_FUSED, s))) return rc;
     if (hd.has_dk) {
       hipLaunchKernelGGL((k_fused<R, RB, TB, true, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h,
                          dO, dC, dX, tiles);
@@ -639,13 +638,101 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
                          dO, dC, dX, tiles);
     }
     QS_CHECK_HIP(hipGetLastError());
-    rc = prof_end(ctx, s);
-    if (rc) return rc;
+    if (prof && (rc = prof_end(ctx, s))) return rc;
     ++npass;
   }
-  if (passes) *passes = npass;
+  if (npass_out) *npass_out = npass;
+  return QS_OK;
+}
+
+template <typename R, int RB, int TB>
+static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int* passes,
+                       hipStream_t s) {
+  const Config c{RB, TB, L};  // LR = 0: low bits stay on threads at the load and store
+  FusedProgram prog;
+  int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
+  if (rc) return rc;
+  if (const char* dbg = getenv("QS_FUSION_DEBUG"))
+    if (dbg[0] == '1') print_program_stats(prog, n, RB + TB, L, ngates);
+  ProgImage im;
+  pack_program<R>(prog, im);
+  rc = ensure_prog(ctx, im.host.size() + 256);
+  if (rc) return rc;
+  char* dprog = (char*)ctx->prog;
+  if (!im.host.empty())
+    QS_CHECK_HIP(hipMemcpyAsync(dprog, im.host.data(), im.host.size(), hipMemcpyHostToDevice, s));
+  rc = launch_passes<R, RB, TB>(ctx, v, n, gates, prog, dprog, im, s, true, passes);
+  if (rc) return rc;
   // the host staging buffer must outlive the async upload
   QS_CHECK_HIP(hipStreamSynchronize(s));
+  return QS_OK;
+}
+
+}  // namespace qs
+
+// A gate sequence planned and uploaded once, its passes captured in a HIP graph.
+struct qs_graph {
+  int device = 0;
+  int passes = 0;
+  void* dprog = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace qs {
+
+template <typename R> __global__ __launch_bounds__(256) void k_basis0(cx<R>* __restrict__ v, uint64_t N) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride)
+    st(v + i, mk<R>(i == 0 ? (R)1 : (R)0, (R)0));
+}
+
+template <typename R, int RB, int TB>
+static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int init_zero,
+                     qs_graph* G) {
+  const Config c{RB, TB, L};
+  FusedProgram prog;
+  int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
+  if (rc) return rc;
+  for (const PassPlan& p : prog.plans)
+    if (!p.fused) {
+      set_error("qs_graph_capture: a gate the tile kernel cannot take (more than 4 register targets) "
+                "would run as its own pass; such sequences are not captured");
+      return QS_ERR_UNSUPPORTED;
+    }
+  ProgImage im;
+  pack_program<R>(prog, im);
+  QS_CHECK_HIP(hipMalloc(&G->dprog, std::max<size_t>(im.host.size(), 256)));
+  if (!im.host.empty()) QS_CHECK_HIP(hipMemcpy(G->dprog, im.host.data(), im.host.size(), hipMemcpyHostToDevice));
+  hipStream_t s = nullptr;
+  QS_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    set_error(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    return QS_ERR_DEVICE;
+  }
+  if (init_zero) {
+    const uint64_t N = 1ull << n;
+    const uint64_t blocks = std::min<uint64_t>((N + 255) / 256, 1ull << 20);
+    hipLaunchKernelGGL(k_basis0<R>, dim3((unsigned)blocks), dim3(256), 0, s, v, N);
+  }
+  int npass = 0;
+  rc = launch_passes<R, RB, TB>(ctx, v, n, gates, prog, (const char*)G->dprog, im, s, false, &npass);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(s, &graph);
+  (void)hipStreamDestroy(s);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return QS_ERR_DEVICE;
+  }
+  G->graph = graph;
+  QS_CHECK_HIP(hipGraphInstantiate(&G->exec, graph, nullptr, nullptr, 0));
+  G->passes = npass;
   return QS_OK;
 }
 
@@ -711,3 +798,78 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 }
 
 }  // namespace qs
+
+namespace qs {
+
+static int graph_capture(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int init_zero,
+                         qs_graph* G) {
+  const int K = ctx->tile_log2 ? ctx->tile_log2 : 12;
+  const int L = ctx->low_bits ? ctx->low_bits : 4;
+  if (n < K + 1) {
+    set_error("qs_graph_capture: registers below 2^(tile+1) amplitudes run one kernel per gate and are not captured");
+    return QS_ERR_UNSUPPORTED;
+  }
+  if (dtype == QS_C128) {
+    cx<double>* v = (cx<double>*)vec;
+    switch (K) {
+      case 11: set_lds_attr<double, 4, 7>(); return capture_t<double, 4, 7>(ctx, v, n, gates, ngates, L, init_zero, G);
+      case 12: set_lds_attr<double, 4, 8>(); return capture_t<double, 4, 8>(ctx, v, n, gates, ngates, L, init_zero, G);
+      case 13: set_lds_attr<double, 4, 9>(); return capture_t<double, 4, 9>(ctx, v, n, gates, ngates, L, init_zero, G);
+      default: set_error("complex128 fusion tiles: tile_log2 must be 11, 12 or 13"); return QS_ERR_UNSUPPORTED;
+    }
+  }
+  cx<float>* v = (cx<float>*)vec;
+  switch (K) {
+    case 12: set_lds_attr<float, 4, 8>(); return capture_t<float, 4, 8>(ctx, v, n, gates, ngates, L, init_zero, G);
+    case 13: set_lds_attr<float, 4, 9>(); return capture_t<float, 4, 9>(ctx, v, n, gates, ngates, L, init_zero, G);
+    default: set_error("complex64 fusion tiles: tile_log2 must be 12 or 13"); return QS_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace qs
+
+using namespace qs;
+
+extern "C" {
+
+int qs_graph_capture(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int init_zero,
+                     qs_graph** out) {
+  QS_REQUIRE(ctx, "null ctx");
+  QS_REQUIRE(out != nullptr, "null output handle");
+  *out = nullptr;
+  int rc = check_state_args(vec, n, dtype);
+  if (rc) return rc;
+  QS_REQUIRE(ngates >= 0 && (ngates == 0 || gates != nullptr), "bad gate list");
+  for (int i = 0; i < ngates; ++i) {
+    rc = validate_gate(&gates[i], n);
+    if (rc) return rc;
+  }
+  QS_CHECK_HIP(hipSetDevice(ctx->device));
+  qs_graph* G = new qs_graph();
+  G->device = ctx->device;
+  rc = graph_capture(ctx, vec, n, dtype, gates, ngates, init_zero, G);
+  if (rc) {
+    qs_graph_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return QS_OK;
+}
+
+int qs_graph_launch(qs_graph* g, void* stream) {
+  QS_REQUIRE(g != nullptr && g->exec != nullptr, "null or empty graph");
+  QS_CHECK_HIP(hipGraphLaunch(g->exec, as_stream(stream)));
+  return QS_OK;
+}
+
+int qs_graph_passes(const qs_graph* g) { return g ? g->passes : -1; }
+
+void qs_graph_destroy(qs_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->dprog) (void)hipFree(g->dprog);
+  delete g;
+}
+
+}  // extern "C"

@@ -29,6 +29,41 @@ from .device_array import DeviceArray
 from .gate_compiler import compile_gates
 
 
+class CapturedCircuit:
+    """A unitary circuit planned, uploaded and captured once in a HIP graph
+    (``qs_graph_capture``).  Every ``replay()`` re-simulates it from |0...0> on
+    its own device vector with no host-side compilation or planning; the
+    simulator that made it can then ``sample`` the result."""
+
+    def __init__(self, sim, circuit: Circuit, tensor, handle, passes: int):
+        self._sim = sim
+        self._circuit = circuit
+        self._tensor = tensor          # the graph writes this buffer: keep it alive
+        self._handle = handle
+        self.passes = passes
+
+    def replay(self) -> DeviceArray:
+        sim = self._sim
+        nat.check(sim._lib.qs_graph_launch(self._handle, sim._stream()), "qs_graph_launch")
+        sim._vector = self._tensor
+        sim._circuit = self._circuit
+        sim._qubits = int(self._circuit.width())
+        if sim._sync:
+            sim._torch.cuda.synchronize(sim._device_id)
+        return DeviceArray(self._tensor)
+
+    def close(self):
+        if self._handle:
+            self._sim._lib.qs_graph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class StateVectorSimulator:
     __DEVICE = ["GPU"]
     __PRECISION = ["single", "double"]
@@ -140,6 +175,25 @@ class StateVectorSimulator:
         if self._sync:
             self._torch.cuda.synchronize(self._device_id)
         return self.vector
+
+    def capture(self, circuit: Circuit) -> CapturedCircuit:
+        """Plan and capture a circuit of unitary gates once; ``replay()`` then runs
+        it from |0...0> as one HIP graph launch (no per-run compilation)."""
+        ops = list(circuit.gates)
+        for op in ops:
+            if not isinstance(op, BasicGate) or op.type in (GateType.measure, GateType.reset, GateType.perm):
+                raise qerr.TypeError("capture takes unitary gates only, got", getattr(op, "type", type(op)))
+        if self._precision != circuit.precision:
+            circuit.convert_precision()
+        n = int(circuit.width())
+        recs, keep = compile_gates(ops, n)
+        tensor = self._torch.empty(1 << n, dtype=self._tdtype, device=f"cuda:{self._device_id}")
+        handle = ctypes.c_void_p(0)
+        rc = self._lib.qs_graph_capture(self._ctx, ctypes.c_void_p(tensor.data_ptr()), n, self._dcode,
+                                        nat.gate_array_ptr(recs), len(recs), 1, ctypes.byref(handle))
+        del keep
+        nat.check(rc, "qs_graph_capture")
+        return CapturedCircuit(self, circuit, tensor, handle, self._lib.qs_graph_passes(handle))
 
     def _flush(self, gates: List[BasicGate]):
         if not gates:
