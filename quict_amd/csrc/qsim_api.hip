@@ -74,19 +74,37 @@ int prof_collect(qs_ctx* ctx) {
   return QS_OK;
 }
 
+// Streaming copy, 16 B per lane: every thread first issues its CP_U loads
+// (strided by the block size, so each load instruction of a wave covers 1 KiB
+// contiguously), then its CP_U stores.
+constexpr int CP_U = 4;
 template <bool NT>
 __global__ __launch_bounds__(256) void k_copy16(const double2* __restrict__ src, double2* __restrict__ dst,
                                                 uint64_t n) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    if constexpr (NT) {
-      double2 v;
-      v.x = __builtin_nontemporal_load(&src[i].x);
-      v.y = __builtin_nontemporal_load(&src[i].y);
-      __builtin_nontemporal_store(v.x, &dst[i].x);
-      __builtin_nontemporal_store(v.y, &dst[i].y);
-    } else {
-      dst[i] = src[i];
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * CP_U) + threadIdx.x;
+  double2 v[CP_U];
+#pragma unroll
+  for (int k = 0; k < CP_U; ++k) {
+    const uint64_t i = base + (uint64_t)k * 256;
+    if (i < n) {
+      if constexpr (NT) {
+        v[k].x = __builtin_nontemporal_load(&src[i].x);
+        v[k].y = __builtin_nontemporal_load(&src[i].y);
+      } else {
+        v[k] = src[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CP_U; ++k) {
+    const uint64_t i = base + (uint64_t)k * 256;
+    if (i < n) {
+      if constexpr (NT) {
+        __builtin_nontemporal_store(v[k].x, &dst[i].x);
+        __builtin_nontemporal_store(v[k].y, &dst[i].y);
+      } else {
+        dst[i] = v[k];
+      }
     }
   }
 }
@@ -341,7 +359,8 @@ int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits) {
 int qs_stream_copy(qs_ctx* ctx, void* dst, const void* src, uint64_t bytes, int nontemporal, void* stream) {
   QS_REQUIRE(ctx && dst && src && bytes % 16 == 0, "bad copy arguments (bytes must be a multiple of 16)");
   const uint64_t n16 = bytes / 16;
-  const uint64_t blocks = std::min<uint64_t>((n16 + 255) / 256, 1ull << 20);
+  const uint64_t blocks = (n16 + 256 * CP_U - 1) / (256 * CP_U);
+  QS_REQUIRE(blocks <= 0x7fffffffull, "copy too large for one launch");
   int rc = prof_begin(ctx, PROF_GATE, as_stream(stream));
   if (rc) return rc;
   if (nontemporal)
