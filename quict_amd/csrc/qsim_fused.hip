@@ -124,9 +124,9 @@ __device__ __forceinline__ void do_dense1(cx<R> (&a)[NR], const cx<R>* m, uint32
 }
 
 // a0' = m0 a1, a1' = m1 a0 (pure: plain exchange) where the controls hold.
-template <typename R, int NR, int P, int CM>
-__device__ __forceinline__ void do_perm1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok,
-                                         int pure) {
+// PURE: plain exchange (X, CX, ...): no phase products at all
+template <typename R, int NR, int P, int CM, bool PURE>
+__device__ __forceinline__ void do_perm1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok) {
   const cx<R> m0 = m[0], m1 = m[1];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -136,7 +136,11 @@ __device__ __forceinline__ void do_perm1(cx<R> (&a)[NR], const cx<R>* m, uint32_
     if constexpr (CM >= 2) on = on && tok;
     const int r1 = r | (1 << P);
     const cx<R> x0 = a[r], x1 = a[r1];
-    const cx<R> y0 = pure ? x1 : mul(m0, x1), y1 = pure ? x0 : mul(m1, x0);
+    cx<R> y0 = x1, y1 = x0;
+    if constexpr (!PURE) {
+      y0 = mul(m0, x1);
+      y1 = mul(m1, x0);
+    }
     a[r] = sel(on, y0, x0);
     a[r1] = sel(on, y1, x1);
   }
@@ -190,9 +194,8 @@ __device__ __forceinline__ void do_xx2(cx<R> (&a)[NR], const cx<R>* m) {
   }
 }
 
-template <typename R, int NR, int P0, int P1, int CM>
-__device__ __forceinline__ void do_swap2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok,
-                                         int pure) {
+template <typename R, int NR, int P0, int P1, int CM, bool PURE>
+__device__ __forceinline__ void do_swap2(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok) {
   const cx<R> m0 = m[0], m1 = m[1];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -202,7 +205,11 @@ __device__ __forceinline__ void do_swap2(cx<R> (&a)[NR], const cx<R>* m, uint32_
     if constexpr (CM >= 2) on = on && tok;
     const int i01 = r | (1 << P1), i10 = r | (1 << P0);
     const cx<R> x01 = a[i01], x10 = a[i10];
-    const cx<R> y01 = pure ? x10 : mul(m0, x10), y10 = pure ? x01 : mul(m1, x01);
+    cx<R> y01 = x10, y10 = x01;
+    if constexpr (!PURE) {
+      y01 = mul(m0, x10);
+      y10 = mul(m1, x01);
+    }
     a[i01] = sel(on, y01, x01);
     a[i10] = sel(on, y10, x10);
   }
@@ -375,7 +382,10 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
       switch (pos0 * 4 + cm) {
 #define QS_P1(P, M) \
   case P * 4 + M: \
-    if constexpr (P < RB) do_perm1<R, NR, P, M>(a, g, creg, cregv, tok, pure); \
+    if constexpr (P < RB) {                                                                   \
+      if (pure) do_perm1<R, NR, P, M, true>(a, g, creg, cregv, tok);                           \
+      else do_perm1<R, NR, P, M, false>(a, g, creg, cregv, tok);                               \
+    }                                                                                         \
     break;
 #define QS_P1P(P) QS_P1(P, 0) QS_P1(P, 1) QS_P1(P, 2) QS_P1(P, 3)
         QS_P1P(0) QS_P1P(1) QS_P1P(2) QS_P1P(3) QS_P1P(4)
@@ -406,8 +416,13 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
   case A * 8 + B:                                                                    \
     if constexpr (A < RB) {                                                          \
       if (sw) {                                                                      \
-        if (ctl) do_swap2<R, NR, A, B, 3>(a, g, creg, cregv, tok, pure);             \
-        else do_swap2<R, NR, A, B, 0>(a, g, creg, cregv, tok, pure);                 \
+        if (ctl) {                                                                   \
+          if (pure) do_swap2<R, NR, A, B, 3, true>(a, g, creg, cregv, tok);          \
+          else do_swap2<R, NR, A, B, 3, false>(a, g, creg, cregv, tok);              \
+        } else {                                                                     \
+          if (pure) do_swap2<R, NR, A, B, 0, true>(a, g, creg, cregv, tok);          \
+          else do_swap2<R, NR, A, B, 0, false>(a, g, creg, cregv, tok);              \
+        }                                                                            \
       } else if (xx) {                                                               \
         if (pure) do_xx2<R, NR, A, B, true>(a, g);                                   \
         else do_xx2<R, NR, A, B, false>(a, g);                                       \
