@@ -96,6 +96,20 @@ template <typename R> __device__ __forceinline__ cx<R> sel(bool c, cx<R> x, cx<R
 template <typename R> __device__ __forceinline__ cx<R> one() { return mk<R>((R)1, (R)0); }
 template <typename R> __device__ __forceinline__ cx<R> zero() { return mk<R>((R)0, (R)0); }
 
+// Uncontrolled dense1 with a real matrix (H, Ry, their products): half the products.
+template <typename R, int NR, int P>
+__device__ __forceinline__ void do_dense1_real(cx<R> (&a)[NR], const cx<R>* m) {
+  const R m00 = m[0].re, m01 = m[1].re, m10 = m[2].re, m11 = m[3].re;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r & (1 << P)) continue;
+    const int r1 = r | (1 << P);
+    const cx<R> x0 = a[r], x1 = a[r1];
+    a[r] = mk<R>(rfma(m01, x1.re, m00 * x0.re), rfma(m01, x1.im, m00 * x0.im));
+    a[r1] = mk<R>(rfma(m11, x1.re, m10 * x0.re), rfma(m11, x1.im, m10 * x0.im));
+  }
+}
+
 template <typename R, int NR, int P, int CM>
 __device__ __forceinline__ void do_dense1(cx<R> (&a)[NR], const cx<R>* m, uint32_t creg, uint32_t cregv, bool tok) {
   cx<R> m00 = m[0], m01 = m[1], m10 = m[2], m11 = m[3];
@@ -368,9 +382,12 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
   switch (kind) {
     case F_DENSE1:
       switch (pos0 * 4 + cm) {
-#define QS_D1(P, M) \
-  case P * 4 + M: \
-    if constexpr (P < RB) do_dense1<R, NR, P, M>(a, g, creg, cregv, tok); \
+#define QS_D1(P, M)                                                   \
+  case P * 4 + M:                                                     \
+    if constexpr (P < RB) {                                           \
+      if (M == 0 && pure) do_dense1_real<R, NR, P>(a, g);              \
+      else do_dense1<R, NR, P, M>(a, g, creg, cregv, tok);             \
+    }                                                                 \
     break;
 #define QS_D1P(P) QS_D1(P, 0) QS_D1(P, 1) QS_D1(P, 2) QS_D1(P, 3)
         QS_D1P(0) QS_D1P(1) QS_D1P(2) QS_D1P(3) QS_D1P(4)

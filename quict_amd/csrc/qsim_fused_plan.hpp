@@ -35,7 +35,8 @@ constexpr int FMAX_PASS_COEF = 1024;
 // One op of a pass program, as the host planner builds it.
 struct FOp {
   int kind;
-  int pure;         // PERM1/SWAP2 without phases; DIAG: lane mode (one value per lane)
+  int pure;         // PERM1/SWAP2 without phases; DIAG: lane mode (one value per lane);
+                    // DENSE1: uncontrolled real matrix; XX2: real payload
   int npos;         // register positions used (targets, MSB first)
   int ntt;          // DIAG: number of selecting thread bits (<= 2)
   int pos[4];
@@ -963,6 +964,12 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
       if (b.op.creg == 0) continue;  // the merged product is the identity
     }
     if (b.op.kind == F_DENSE3 || b.op.kind == F_DENSE4) H.has_dk = 1;
+    if (b.op.kind == F_DENSE1) {
+      // uncontrolled with a real (merged) matrix: the kernel's half-cost form
+      bool real = b.op.creg == 0 && b.op.cthr == 0;
+      for (size_t i = 1; i < b.coef.size(); i += 2) real = real && b.coef[i] == 0.0;
+      b.op.pure = real;
+    }
     if (b.op.kind != F_RELAYOUT) {
       b.op.coef = (int)(prog.coef.size() / 2) - coef_start;
       prog.coef.insert(prog.coef.end(), b.coef.begin(), b.coef.end());

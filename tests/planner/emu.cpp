@@ -115,6 +115,11 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
         switch (op.kind) {
           case F_DENSE1: {
             const int P0 = op.pos[0];
+            if (op.pure) {
+              if (op.creg || op.cthr) throw std::runtime_error("real dense1 with controls");
+              for (int k = 0; k < 4; ++k)
+                if (cz(c0 + k).imag() != 0.0) throw std::runtime_error("real dense1 payload with imaginary part");
+            }
             for (int r = 0; r < NR; ++r) {
               if ((r >> P0) & 1 || !regok(r) || !tok) continue;
               const int r1 = r | (1 << P0);
