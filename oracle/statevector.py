@@ -22,7 +22,10 @@ Deliberate differences from the reference, each a reference defect:
     golden circuit or the benchmark workloads;
   * iSwap-family phases are applied (the reference CPU swap ignores them);
   * a diagonal gate as wide as the register is applied (the reference CPU
-    path rebinds a local and drops it).
+    path rebinds a local and drops it);
+  * a Trigger whose mapping yields a single BasicGate applies that gate (the
+    reference accepts such mappings, trigger.py:33-57, but only splices
+    CompositeGates, statevector_simulator.py:514-523, so it drops them).
 Sampling uses CPython's ``random.random()`` — the generator the reference's
 GPU path and density-matrix path draw from (gate_function.py:1795,
 density_matrix_simulator.py:191).

@@ -292,6 +292,9 @@ class StateVectorSimulator:
         cg = op.mapping(int(self._circuit.qubits[op.targs]))
         if cg is None:
             return
+        if getattr(cg, "checkpoint", None) is not None:
+            raise qerr.GateTypeNotImplementError("Trigger mappings with a checkpoint position")
+        # a single BasicGate is applied too (the reference silently drops it)
         gates = cg.gates if isinstance(cg, CompositeGate) else [cg]
         self._pipeline = self._pipeline[:current_idx] + list(gates) + self._pipeline[current_idx:]
 
