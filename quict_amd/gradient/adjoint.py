@@ -87,6 +87,7 @@ class ParameterizedCircuit:
         self.n = n
         self.ops: List[Tuple[GateType, List[int], Optional[int], Optional[list]]] = []
         self.nparams = 0
+        self._template = None   # record templates, rebuilt after every add()
 
     def add(self, gtype: GateType, qubits, param: Optional[int] = None, value=None):
         qubits = [qubits] if isinstance(qubits, int) else list(qubits)
@@ -96,6 +97,7 @@ class ParameterizedCircuit:
             self.nparams = max(self.nparams, param + 1)
         vals = None if value is None else (list(value) if isinstance(value, (list, tuple)) else [value])
         self.ops.append((gtype, qubits, param, vals))
+        self._template = None
         return self
 
     @staticmethod
@@ -110,19 +112,21 @@ class ParameterizedCircuit:
                 pc.add(GateType.cx, [q, q + 1])
         return pc
 
+    def gate(self, j: int, theta: np.ndarray):
+        """Gate object of op j at angles theta."""
+        gtype, qubits, param, vals = self.ops[j]
+        g = GATE_TYPE_TO_CLASS[gtype]()
+        if param is not None:
+            g = g(float(theta[param]))
+        elif vals is not None:
+            g = g(*vals)
+        c = g.controls
+        g.cargs = qubits[:c]
+        g.targs = qubits[c:]
+        return g
+
     def gates(self, theta: np.ndarray):
-        out = []
-        for gtype, qubits, param, vals in self.ops:
-            g = GATE_TYPE_TO_CLASS[gtype]()
-            if param is not None:
-                g = g(float(theta[param]))
-            elif vals is not None:
-                g = g(*vals)
-            c = g.controls
-            g.cargs = qubits[:c]
-            g.targs = qubits[c:]
-            out.append((g, param))
-        return out
+        return [(self.gate(j, theta), self.ops[j][2]) for j in range(len(self.ops))]
 
     def bind(self, theta: np.ndarray) -> Circuit:
         c = Circuit(self.n)
@@ -145,7 +149,86 @@ def _derivative_target_block(g) -> np.ndarray:
     raise qerr.ValueError(f"no derivative rule for {gt.name}")
 
 
+def _rotation_payloads(gtype: GateType, th: np.ndarray):
+    """(forward, inverse, derivative) record payloads of Rx / Ry / Rz for the
+    angles th, one row per gate: the target blocks of exp(-i th P / 2), of its
+    conjugate transpose and of d/dth = -(i/2) P exp(-i th P / 2)
+    (QuICT/core/gate/gate.py:963-1096).  Rz records are diagonal."""
+    c, s = np.cos(th / 2), np.sin(th / 2)
+    if gtype == GateType.ry:
+        fwd = np.stack([c, -s, s, c], 1).astype(np.complex128)
+        inv = np.stack([c, s, -s, c], 1).astype(np.complex128)
+        der = np.stack([-s / 2, -c / 2, c / 2, -s / 2], 1).astype(np.complex128)
+    elif gtype == GateType.rx:
+        fwd = np.stack([c, -1j * s, -1j * s, c], 1)
+        inv = np.stack([c, 1j * s, 1j * s, c], 1)
+        der = np.stack([-s / 2 + 0j, -0.5j * c, -0.5j * c, -s / 2 + 0j], 1)
+    else:
+        e0, e1 = np.exp(-0.5j * th), np.exp(0.5j * th)
+        z = np.zeros_like(e0)
+        fwd = np.stack([e0, e1], 1)
+        inv = np.stack([e1, e0], 1)
+        der = np.stack([-0.5j * e0, z, z, 0.5j * e1], 1)
+    return fwd, inv, der
+
+
+_FAST_TYPES = (GateType.rx, GateType.ry, GateType.rz)
+_PROBE_ANGLE = 0.7317
+
+
+def _template(pc: ParameterizedCircuit):
+    """Records of pc built once per circuit structure by the generic compiler
+    (at a probe angle), plus, per rotation type whose records the closed forms
+    above reproduce exactly, the rows and parameter indices to refill per call."""
+    if pc._template is not None:
+        return pc._template
+    probe = np.full(max(pc.nparams, 1), _PROBE_ANGLE)
+    fwd, inv, der, slot, keep = _records_generic(pc, probe)
+    fast = {}
+    for gt in _FAST_TYPES:
+        rows = [j for j, (t, _, p, _) in enumerate(pc.ops) if t == gt and p is not None]
+        if not rows:
+            continue
+        rows = np.array(rows)
+        f, i, d = _rotation_payloads(gt, np.full(len(rows), _PROBE_ANGLE))
+        ok = True
+        for arr, pay in ((fwd, f), (inv, i), (der, d)):
+            got = arr["m"][rows][:, : 2 * pay.shape[1]].view(np.complex128)
+            ok = ok and np.array_equal(got, pay) and not np.any(arr["m"][rows][:, 2 * pay.shape[1]:])
+        if ok:
+            fast[gt] = (rows, np.array([pc.ops[j][2] for j in rows]))
+    generic = [j for j, (t, _, p, _) in enumerate(pc.ops)
+               if p is not None and not any(t == gt for gt in fast)]
+    pc._template = (fwd, inv, der, slot, keep, fast, generic)
+    return pc._template
+
+
 def _records(pc: ParameterizedCircuit, theta: np.ndarray):
+    """Forward, inverse and derivative records of pc at theta: the structure's
+    template with the rotation payloads refilled in closed form (vectorised);
+    other parameterised gates go through the generic compiler."""
+    fwd0, inv0, der0, slot, keep0, fast, generic = _template(pc)
+    fwd, inv, der = fwd0.copy(), inv0.copy(), der0.copy()
+    keep = list(keep0)
+    for gt, (rows, params) in fast.items():
+        f, i, d = _rotation_payloads(gt, np.asarray(theta, dtype=np.float64)[params])
+        for arr, pay in ((fwd, f), (inv, i), (der, d)):
+            m = arr["m"][rows]
+            m[:, : 2 * pay.shape[1]] = pay.view(np.float64)
+            arr["m"][rows] = m
+    if generic:
+        for j in generic:
+            g = pc.gate(j, theta)
+            compile_gate(g, pc.n, fwd[j], keep)
+            compile_gate(g, pc.n, inv[j], keep, dagger=True)
+            D = _derivative_target_block(g)
+            der[j]["m"][:] = 0
+            v = np.ascontiguousarray(D.ravel())
+            der[j]["m"][: 2 * len(v)] = v.view(np.float64)
+    return fwd, inv, der, slot, keep
+
+
+def _records_generic(pc: ParameterizedCircuit, theta: np.ndarray):
     n = pc.n
     pairs = pc.gates(theta)
     fwd = np.zeros(len(pairs), dtype=nat.QS_GATE_DTYPE)
