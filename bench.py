@@ -398,7 +398,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the simulator state lives on a single MI355X; "
-                                   "multi-GPU sharding is designed (DESIGN.md) but not built; run with --gpus 1"}))
+                                   "sharding over several GPUs is designed but not built; run with --gpus 1"}))
         sys.exit(2)
 
     import torch
