@@ -133,10 +133,59 @@ def compile_gate(gate: BasicGate, n: int, rec: np.ndarray, keep: list, row_offse
     return True
 
 
+# Parameter-free gate types of fixed shape: the record (kind, controls, targets,
+# payload) depends only on the type and precision, so it is translated once and
+# reused; only the bit positions differ between instances.
+_STATIC_TYPES = frozenset([
+    GateType.h, GateType.hy, GateType.s, GateType.sdg, GateType.x, GateType.y, GateType.z, GateType.sx,
+    GateType.sy, GateType.sw, GateType.t, GateType.tdg, GateType.cx, GateType.cy, GateType.cz, GateType.ch,
+    GateType.swap, GateType.iswap, GateType.iswapdg, GateType.sqiswap, GateType.cswap, GateType.ccx,
+    GateType.ccz])
+_STATIC_RECORDS = {}
+# One-angle / three-angle rotations whose payloads are computed for all gates of
+# a call at once with the closed forms of the gate module (core/gate.py).
+_ROTATION_TYPES = frozenset([GateType.rx, GateType.ry, GateType.rz, GateType.crz, GateType.u1, GateType.u3])
+
+
+def _rotation_records(gt: GateType, P: np.ndarray, dt):
+    """(kind per gate, payload rows, degenerate mask) for rotations of type gt with
+    parameter rows P, matching _translate: the matrix in the gate's precision dt,
+    the same record-kind decisions.  Degenerate gates (a dense matrix that is
+    exactly diagonal) go back to the generic path."""
+    G = P.shape[0]
+    cast = (lambda a: a.astype(dt).astype(np.complex128))
+    if gt in (GateType.rx, GateType.ry, GateType.u3):
+        th = P[:, 0]
+        c, s = np.cos(th / 2), np.sin(th / 2)
+        if gt == GateType.rx:
+            pay = np.stack([c + 0j, 1j * -s, 1j * -s, c + 0j], 1)
+        elif gt == GateType.ry:
+            pay = np.stack([c + 0j, -s + 0j, s + 0j, c + 0j], 1)
+        else:
+            pay = np.stack([np.cos(th / 2) + 0j, -np.exp(1j * P[:, 2]) * np.sin(th / 2),
+                            np.exp(1j * P[:, 1]) * np.sin(th / 2), np.exp(1j * (P[:, 1] + P[:, 2])) * np.cos(th / 2)],
+                           1)
+        pay = cast(pay)
+        degenerate = (pay[:, 1] == 0) & (pay[:, 2] == 0)
+        return np.full(G, nat.GATE_DENSE, dtype=np.int32), pay, degenerate
+    if gt == GateType.u1:
+        pay = cast(np.exp(1j * P[:, 0]))[:, None]
+        return np.full(G, nat.GATE_PHASE, dtype=np.int32), pay, np.zeros(G, dtype=bool)
+    # rz, crz: diagonal (e^{-i th/2}, e^{i th/2}); PHASE when the first entry is 1
+    d = cast(np.stack([np.exp(-P[:, 0] / 2 * 1j), np.exp(P[:, 0] / 2 * 1j)], 1))
+    phase = d[:, 0] == 1
+    kind = np.where(phase, nat.GATE_PHASE, nat.GATE_DIAG).astype(np.int32)
+    pay = np.where(phase[:, None], np.stack([d[:, 1], np.zeros(G, dtype=np.complex128)], 1), d)
+    return kind, pay, np.zeros(G, dtype=bool)
+
+
 def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: bool = False,
                   dagger: bool = False) -> Tuple[np.ndarray, list]:
     """Records for a run of unitary gates; ``keep`` holds arrays referenced by pointer.
-    The fields are gathered in plain arrays and written column-wise."""
+    The fields are gathered in plain arrays and written column-wise.  Fixed gate
+    types reuse one translated record per type; Rx / Ry / Rz / CRz / U1 / U3
+    payloads are computed for the whole run at once (plain records only: no row
+    offset, conjugation or inverse); everything else goes through _translate."""
     keep: list = []
     G = len(gates)
     kind = np.zeros(G, dtype=np.int32)
@@ -145,8 +194,34 @@ def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: boo
     bits = np.zeros((G, nat.QS_MAX_GATE_BITS), dtype=np.int32)
     m = np.zeros((G, 32), dtype=np.float64)
     big = np.zeros(G, dtype=np.uint64)
+    fast = row_offset == 0 and not conj and not dagger
+    rot = {}   # (type, precision) -> ([slot], [gate])
     k = 0
     for g in gates:
+        gt = g.type
+        if fast and gt in _STATIC_TYPES:
+            key = (gt, g.precision)
+            rec = _STATIC_RECORDS.get(key)
+            if rec is None:
+                r = _translate(g, n, keep, 0, False, False)
+                rec = _STATIC_RECORDS[key] = (r[0], r[1], r[2], None if r[4] is None else
+                                              np.ascontiguousarray(r[4], dtype=np.complex128).view(np.float64))
+            kind[k], nc[k], nt[k] = rec[0], rec[1], rec[2]
+            args = g.cargs + g.targs
+            bits[k, : len(args)] = [n - 1 - a for a in args]
+            if rec[3] is not None:
+                m[k, : len(rec[3])] = rec[3]
+            k += 1
+            continue
+        if fast and gt in _ROTATION_TYPES:
+            args = g.cargs + g.targs
+            nc[k], nt[k] = g.controls, g.targets
+            bits[k, : len(args)] = [n - 1 - a for a in args]
+            lst = rot.setdefault((gt, g.precision), ([], []))
+            lst[0].append(k)
+            lst[1].append(g)
+            k += 1
+            continue
         r = _translate(g, n, keep, row_offset, conj, dagger)
         if r is None:
             continue
@@ -157,6 +232,21 @@ def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: boo
             m[k, : len(v)] = v
         big[k] = r[5]
         k += 1
+    for (gt, dt), (slots, gl) in rot.items():
+        slots = np.asarray(slots)
+        P = np.array([g.pargs for g in gl], dtype=np.float64)
+        kk, pay, degenerate = _rotation_records(gt, P, dt)
+        kind[slots] = kk
+        m[slots, : 2 * pay.shape[1]] = np.ascontiguousarray(pay).view(np.float64)
+        for i in np.nonzero(degenerate)[0]:
+            r = _translate(gl[i], n, keep, 0, False, False)
+            j = slots[i]
+            kind[j], nc[j], nt[j] = r[0], r[1], r[2]
+            bits[j, :] = 0
+            bits[j, : len(r[3])] = r[3]
+            m[j, :] = 0
+            v = np.ascontiguousarray(r[4], dtype=np.complex128).view(np.float64)
+            m[j, : len(v)] = v
     arr = np.zeros(k, dtype=nat.QS_GATE_DTYPE)
     arr["kind"] = kind[:k]
     arr["ncontrols"] = nc[:k]
