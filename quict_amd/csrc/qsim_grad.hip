@@ -87,6 +87,175 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_step(cx<R>* __restrict__ ps
   }
 }
 
+// ---- several consecutive steps in one pass ------------------------------------
+// A run of gates (in backward order) whose bits together span at most NB
+// amplitude bits U: each thread loads groups of 2^NB amplitudes of psi and lam
+// (the bits of U varying), applies every step of the run in registers and
+// stores once.  Gate positions are local bit indices into U.
+constexpr int MG_MAX_GATES = 8;
+struct MGate {
+  int32_t nt;          // 1 or 2 targets
+  int32_t p0, p1;      // local positions of target 0 (matrix MSB) and target 1
+  int32_t grad;        // parameterised: accumulate into partial row `row`
+  int32_t row;
+  uint32_t cmask, cval;  // local control bits and the values they need
+  int32_t pad;
+  double g[32];        // G^dagger, (re, im), 2^nt x 2^nt
+  double d[32];        // dG/dtheta
+};
+
+template <typename R> __device__ __forceinline__ cx<R> cget(const double* m, int i) {
+  return mk<R>((R)m[2 * i], (R)m[2 * i + 1]);
+}
+
+template <typename R, int NG, int P, bool GRAD>
+__device__ __forceinline__ void mg_apply1(cx<R> (&p)[NG], cx<R> (&l)[NG], const MGate& G, double& acc) {
+  const cx<R> g00 = cget<R>(G.g, 0), g01 = cget<R>(G.g, 1), g10 = cget<R>(G.g, 2), g11 = cget<R>(G.g, 3);
+#pragma unroll
+  for (int i = 0; i < NG; ++i) {
+    if (i & (1 << P)) continue;
+    const int i1 = i | (1 << P);
+    if (((uint32_t)i & G.cmask) != G.cval) continue;
+    const cx<R> a0 = cfma(mul(g00, p[i]), g01, p[i1]), a1 = cfma(mul(g10, p[i]), g11, p[i1]);
+    const cx<R> b0 = cfma(mul(g00, l[i]), g01, l[i1]), b1 = cfma(mul(g10, l[i]), g11, l[i1]);
+    if constexpr (GRAD) {
+      const cx<R> m0 = cfma(mul(cget<R>(G.d, 0), a0), cget<R>(G.d, 1), a1);
+      const cx<R> m1 = cfma(mul(cget<R>(G.d, 2), a0), cget<R>(G.d, 3), a1);
+      acc += (double)l[i].re * (double)m0.re + (double)l[i].im * (double)m0.im +
+             (double)l[i1].re * (double)m1.re + (double)l[i1].im * (double)m1.im;
+    }
+    p[i] = a0;
+    p[i1] = a1;
+    l[i] = b0;
+    l[i1] = b1;
+  }
+}
+
+template <typename R, int NG, int P0, int P1, bool GRAD>
+__device__ __forceinline__ void mg_apply2(cx<R> (&p)[NG], cx<R> (&l)[NG], const MGate& G, double& acc) {
+#pragma unroll
+  for (int i = 0; i < NG; ++i) {
+    if (i & ((1 << P0) | (1 << P1))) continue;
+    if (((uint32_t)i & G.cmask) != G.cval) continue;
+    const int idx[4] = {i, i | (1 << P1), i | (1 << P0), i | (1 << P0) | (1 << P1)};
+    cx<R> a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      cx<R> sa = mul(cget<R>(G.g, 4 * r), p[idx[0]]), sb = mul(cget<R>(G.g, 4 * r), l[idx[0]]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) {
+        sa = cfma(sa, cget<R>(G.g, 4 * r + c), p[idx[c]]);
+        sb = cfma(sb, cget<R>(G.g, 4 * r + c), l[idx[c]]);
+      }
+      a[r] = sa;
+      b[r] = sb;
+    }
+    if constexpr (GRAD) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cx<R> mu = mul(cget<R>(G.d, 4 * r), a[0]);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) mu = cfma(mu, cget<R>(G.d, 4 * r + c), a[c]);
+        acc += (double)l[idx[r]].re * (double)mu.re + (double)l[idx[r]].im * (double)mu.im;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      p[idx[r]] = a[r];
+      l[idx[r]] = b[r];
+    }
+  }
+}
+
+template <typename R, int NB, bool GRAD>
+__device__ __forceinline__ void mg_apply(cx<R> (&p)[1 << NB], cx<R> (&l)[1 << NB], const MGate& G, double& acc) {
+  constexpr int NG = 1 << NB;
+  if (G.nt == 1) {
+    switch (G.p0) {
+      case 0: mg_apply1<R, NG, 0, GRAD>(p, l, G, acc); break;
+      case 1: if constexpr (NB > 1) mg_apply1<R, NG, 1, GRAD>(p, l, G, acc); break;
+      case 2: if constexpr (NB > 2) mg_apply1<R, NG, 2, GRAD>(p, l, G, acc); break;
+      default: break;
+    }
+  } else {
+    switch (G.p0 * 4 + G.p1) {
+      case 1: if constexpr (NB > 1) mg_apply2<R, NG, 0, 1, GRAD>(p, l, G, acc); break;
+      case 4: if constexpr (NB > 1) mg_apply2<R, NG, 1, 0, GRAD>(p, l, G, acc); break;
+      case 2: if constexpr (NB > 2) mg_apply2<R, NG, 0, 2, GRAD>(p, l, G, acc); break;
+      case 8: if constexpr (NB > 2) mg_apply2<R, NG, 2, 0, GRAD>(p, l, G, acc); break;
+      case 6: if constexpr (NB > 2) mg_apply2<R, NG, 1, 2, GRAD>(p, l, G, acc); break;
+      case 9: if constexpr (NB > 2) mg_apply2<R, NG, 2, 1, GRAD>(p, l, G, acc); break;
+      default: break;
+    }
+  }
+}
+
+struct MPass {
+  int32_t ubits[3];   // amplitude bits of U, ascending (local bit b = ubits[b])
+  int32_t nb;
+  int32_t g0, ng;     // gates [g0, g0 + ng) of the descriptor table, in application order
+};
+
+template <typename R, int NB>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_multi(cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                           uint64_t ngroups, const MPass P,
+                                                           const MGate* __restrict__ gates,
+                                                           double* __restrict__ partial) {
+  constexpr int NG = 1 << NB;
+  __shared__ double red[MG_MAX_GATES][ADJ_THREADS / 64];
+  double acc[MG_MAX_GATES];
+#pragma unroll
+  for (int k = 0; k < MG_MAX_GATES; ++k) acc[k] = 0.0;
+  uint64_t off[NG];
+#pragma unroll
+  for (int i = 0; i < NG; ++i) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if ((i >> b) & 1) o |= 1ull << P.ubits[b];
+    off[i] = o;
+  }
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t base = insert_zeros(g, P.ubits, NB);
+    cx<R> p[NG], l[NG];
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+      p[i] = ld(psi + (base | off[i]));
+      l[i] = ld(lam + (base | off[i]));
+    }
+#pragma unroll
+    for (int k = 0; k < MG_MAX_GATES; ++k) {
+      if (k >= P.ng) break;
+      const MGate& G = gates[P.g0 + k];
+      if (G.grad) mg_apply<R, NB, true>(p, l, G, acc[k]);
+      else mg_apply<R, NB, false>(p, l, G, acc[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < NG; ++i) {
+      st(psi + (base | off[i]), p[i]);
+      st(lam + (base | off[i]), l[i]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < MG_MAX_GATES; ++k) {
+    if (k >= P.ng) break;
+    double a = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < MG_MAX_GATES && (int)threadIdx.x < P.ng) {
+    const MGate& G = gates[P.g0 + threadIdx.x];
+    if (G.grad) {
+      double s = 0;
+      for (int w = 0; w < ADJ_THREADS / 64; ++w) s += red[threadIdx.x][w];
+      partial[(uint64_t)G.row * ADJ_MAX_BLOCKS + blockIdx.x] = s;
+    }
+  }
+}
+
 // sums[k] = sum_b partial[k * stride + b], b < nb[k]
 __global__ void k_adj_reduce(const double* __restrict__ partial, const int* __restrict__ nb, int stride, int nsteps,
                              double* __restrict__ sums) {
@@ -137,30 +306,136 @@ static void record_dense(const qs_gate& g, std::vector<double>& M) {
 template <typename R>
 static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* gates, int ngates,
                      const qs_gate* dgates, const int32_t* slot, double* grads, int nslots, hipStream_t s) {
-  // one partial row per parameterised step
-  int nsteps = 0;
-  for (int j = 0; j < ngates; ++j)
-    if (slot[j] >= 0) ++nsteps;
-  const uint64_t need = (uint64_t)std::max(nsteps, 1) * ADJ_MAX_BLOCKS * sizeof(double) +
-                        (uint64_t)std::max(nsteps, 1) * (sizeof(double) + sizeof(int)) + 1024;
-  int rc = ensure_scratch(ctx, need);
-  if (rc) return rc;
-  double* partial = (double*)ctx->scratch;
-  double* sums = partial + (uint64_t)std::max(nsteps, 1) * ADJ_MAX_BLOCKS;
-  int* d_nb = (int*)(sums + std::max(nsteps, 1));
+  int rc = QS_OK;
   std::vector<int> nb;
   std::vector<int> step_slot;
+  // Plan: runs of consecutive steps (backward order) spanning <= 3 amplitude
+  // bits become one multi-gate pass; other gates run as single steps.
+  struct Item {
+    int multi;   // 1: pass index into passes, 0: single gate index
+    int idx;
+  };
+  std::vector<Item> items;
+  std::vector<MPass> passes;
+  std::vector<MGate> mg;
+  std::vector<int> cur;  // gate indices of the open pass, in application order
+  uint64_t curbits = 0;
+  auto eligible = [&](int j) {
+    const qs_gate& g = gates[j];
+    if (g.ntargets > 2 || g.ncontrols + g.ntargets > 3) return false;
+    if (slot[j] >= 0 && (dgates[j].ntargets != g.ntargets || dgates[j].ncontrols != g.ncontrols)) return false;
+    return true;
+  };
+  auto gbits = [&](int j) {
+    uint64_t m = 0;
+    for (int i = 0; i < gates[j].ncontrols + gates[j].ntargets; ++i) m |= 1ull << gates[j].bits[i];
+    return m;
+  };
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    MPass P;
+    std::memset(&P, 0, sizeof(P));
+    int nbits = 0;
+    for (int b = 0; b < 64; ++b)
+      if ((curbits >> b) & 1) P.ubits[nbits++] = b;
+    P.nb = nbits;
+    P.g0 = (int)mg.size();
+    P.ng = (int)cur.size();
+    const uint64_t ngroups = 1ull << (n - nbits);
+    const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, (ngroups + ADJ_THREADS - 1) / ADJ_THREADS);
+    auto local = [&](int bit) {
+      for (int i = 0; i < nbits; ++i)
+        if (P.ubits[i] == bit) return i;
+      return -1;
+    };
+    for (int j : cur) {
+      const qs_gate& g = gates[j];
+      MGate G;
+      std::memset(&G, 0, sizeof(G));
+      G.nt = g.ntargets;
+      G.p0 = local(g.bits[g.ncontrols]);
+      G.p1 = g.ntargets > 1 ? local(g.bits[g.ncontrols + 1]) : 0;
+      for (int i = 0; i < g.ncontrols; ++i) {
+        G.cmask |= 1u << local(g.bits[i]);
+        G.cval |= 1u << local(g.bits[i]);
+      }
+      std::vector<double> M;
+      record_dense(g, M);
+      for (size_t i = 0; i < M.size(); ++i) G.g[i] = M[i];
+      if (slot[j] >= 0) {
+        record_dense(dgates[j], M);
+        for (size_t i = 0; i < M.size(); ++i) G.d[i] = M[i];
+        G.grad = 1;
+        G.row = (int)nb.size();
+        nb.push_back(blocks);
+        step_slot.push_back(slot[j]);
+      }
+      mg.push_back(G);
+    }
+    items.push_back({1, (int)passes.size()});
+    passes.push_back(P);
+    cur.clear();
+    curbits = 0;
+  };
+  std::vector<int> single_row(ngates, -1);
+  for (int j = ngates - 1; j >= 0; --j) {
+    if (!eligible(j)) {
+      flush();
+      if (slot[j] >= 0) {
+        const qs_gate& g = gates[j];
+        if (g.ntargets > 2 || dgates[j].ntargets != g.ntargets || dgates[j].ncontrols != g.ncontrols) {
+          set_error("adjoint gradient: parameterised gates must act on <= 2 targets");
+          return QS_ERR_UNSUPPORTED;
+        }
+        const uint64_t ngroups = 1ull << (n - g.ncontrols - g.ntargets);
+        single_row[j] = (int)nb.size();
+        nb.push_back((int)std::min<uint64_t>(ADJ_MAX_BLOCKS, (ngroups + ADJ_THREADS - 1) / ADJ_THREADS));
+        step_slot.push_back(slot[j]);
+      }
+      items.push_back({0, j});
+      continue;
+    }
+    const uint64_t b = gbits(j);
+    if (!cur.empty() && (__builtin_popcountll(curbits | b) > 3 || (int)cur.size() >= MG_MAX_GATES)) flush();
+    cur.push_back(j);
+    curbits |= b;
+  }
+  flush();
+  // descriptors after the partial rows, sums and counts
+  const uint64_t rows = std::max<size_t>(nb.size(), 1);
+  const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * (sizeof(double) + sizeof(int)) + 1024 +
+                        mg.size() * sizeof(MGate) + 256;
+  rc = ensure_scratch(ctx, need);
+  if (rc) return rc;
+  double* partial = (double*)ctx->scratch;
+  double* sums = partial + rows * ADJ_MAX_BLOCKS;
+  int* d_nb = (int*)(sums + rows);
+  MGate* d_mg = (MGate*)(((uintptr_t)(d_nb + rows) + 255) & ~(uintptr_t)255);
+  if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_mg, mg.data(), mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
   rc = prof_begin(ctx, PROF_GRAD, s);
   if (rc) return rc;
-  for (int j = ngates - 1; j >= 0; --j) {
+  for (const Item& it : items) {
+    if (it.multi) {
+      const MPass& P = passes[it.idx];
+      const uint64_t ngroups = 1ull << (n - P.nb);
+      const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, (ngroups + ADJ_THREADS - 1) / ADJ_THREADS);
+      if (P.nb == 1)
+        hipLaunchKernelGGL((k_adj_multi<R, 1>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, P, d_mg,
+                           partial);
+      else if (P.nb == 2)
+        hipLaunchKernelGGL((k_adj_multi<R, 2>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, P, d_mg,
+                           partial);
+      else
+        hipLaunchKernelGGL((k_adj_multi<R, 3>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, P, d_mg,
+                           partial);
+      QS_CHECK_HIP(hipGetLastError());
+      continue;
+    }
+    const int j = it.idx;
     const qs_gate& g = gates[j];
     const bool grad = slot[j] >= 0;
     const int t = g.ntargets, c = g.ncontrols;
-    if (t > 2 || (grad && (dgates[j].ntargets != t || dgates[j].ncontrols != c))) {
-      if (grad) {
-        set_error("adjoint gradient: parameterised gates must act on <= 2 targets");
-        return QS_ERR_UNSUPPORTED;
-      }
+    if (t > 2) {
       rc = apply_gate(ctx, psi, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &g, s);
       if (rc) return rc;
       rc = apply_gate(ctx, lam, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &g, s);
@@ -190,7 +465,7 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     }
     const uint64_t ngroups = 1ull << (n - c - t);
     const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, (ngroups + ADJ_THREADS - 1) / ADJ_THREADS);
-    double* prow = partial + (uint64_t)nb.size() * ADJ_MAX_BLOCKS;
+    double* prow = partial + (uint64_t)(grad ? single_row[j] : 0) * ADJ_MAX_BLOCKS;
     if (t == 1) {
       if (grad) hipLaunchKernelGGL((k_adj_step<R, 1, true>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, a, prow);
       else hipLaunchKernelGGL((k_adj_step<R, 1, false>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, a, prow);
@@ -199,10 +474,6 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
       else hipLaunchKernelGGL((k_adj_step<R, 2, false>), dim3(blocks), dim3(ADJ_THREADS), 0, s, psi, lam, ngroups, a, prow);
     }
     QS_CHECK_HIP(hipGetLastError());
-    if (grad) {
-      nb.push_back(blocks);
-      step_slot.push_back(slot[j]);
-    }
   }
   if (!nb.empty()) {
     QS_CHECK_HIP(hipMemcpyAsync(d_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice, s));
