@@ -500,8 +500,51 @@ static bool merge_diag(Built& e, const Built& d, int NR) {
 
 static bool uncontrolled(const FOp& o) { return o.creg == 0 && o.cthr == 0; }
 
+// Dense 4x4 form ((re, im) pairs) of an uncontrolled two-target op.
+static void two_target_dense(const Built& b, double* M) {
+  if (b.op.kind == F_DENSE2) {
+    for (int i = 0; i < 32; ++i) M[i] = b.coef[i];
+    return;
+  }
+  for (int i = 0; i < 32; ++i) M[i] = 0.0;
+  for (int s = 0; s < 4; ++s) {  // F_XX2: diagonal d_s, anti-diagonal o_s
+    M[2 * (5 * s)] = b.coef[2 * s];
+    M[2 * (5 * s) + 1] = b.coef[2 * s + 1];
+    M[2 * (4 * s + (s ^ 3))] = b.coef[2 * (4 + s)];
+    M[2 * (4 * s + (s ^ 3)) + 1] = b.coef[2 * (4 + s) + 1];
+  }
+}
+
+// Store the product of two same-position two-target ops in `e`, in the
+// diagonal + anti-diagonal form when its entries allow.
+static void set_two_target(Built& e, const double* M) {
+  bool xx = true, real = true;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const double re = M[2 * (4 * i + j)], im = M[2 * (4 * i + j) + 1];
+      if ((i ^ j) == 1 || (i ^ j) == 2) xx = xx && re == 0.0 && im == 0.0;
+      else real = real && im == 0.0;
+    }
+  if (xx) {
+    e.op.kind = F_XX2;
+    e.op.pure = real;
+    e.coef.assign(16, 0.0);
+    for (int s = 0; s < 4; ++s) {
+      e.coef[2 * s] = M[2 * (5 * s)];
+      e.coef[2 * s + 1] = M[2 * (5 * s) + 1];
+      e.coef[2 * (4 + s)] = M[2 * (4 * s + (s ^ 3))];
+      e.coef[2 * (4 + s) + 1] = M[2 * (4 * s + (s ^ 3)) + 1];
+    }
+  } else {
+    e.op.kind = F_DENSE2;
+    e.op.pure = 0;
+    e.coef.assign(M, M + 32);
+  }
+}
+
 static bool try_merge(std::vector<Built>& out, Built& d, int NR, int RB) {
-  if (d.op.kind != F_DIAG && d.op.kind != F_DENSE1) return false;
+  const bool two = (d.op.kind == F_DENSE2 || d.op.kind == F_XX2) && uncontrolled(d.op);
+  if (d.op.kind != F_DIAG && d.op.kind != F_DENSE1 && !two) return false;
   int k = (int)out.size() - 1;
   for (; k >= 0; --k) {
     if (out[k].op.kind == F_RELAYOUT) return false;
@@ -509,6 +552,29 @@ static bool try_merge(std::vector<Built>& out, Built& d, int NR, int RB) {
   }
   if (k < 0) return false;
   Built& e = out[k];
+  if (two) {
+    // two-target x two-target on the same register positions: one product op
+    if ((e.op.kind != F_DENSE2 && e.op.kind != F_XX2) || !uncontrolled(e.op) || e.op.pos[0] != d.op.pos[0] ||
+        e.op.pos[1] != d.op.pos[1])
+      return false;
+    double A[32], B[32], P[32];
+    two_target_dense(d, A);
+    two_target_dense(e, B);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < 4; ++t) {
+          const double ar = A[2 * (4 * i + t)], ai = A[2 * (4 * i + t) + 1];
+          const double br = B[2 * (4 * t + j)], bi = B[2 * (4 * t + j) + 1];
+          re += ar * br - ai * bi;
+          im += ar * bi + ai * br;
+        }
+        P[2 * (4 * i + j)] = re;
+        P[2 * (4 * i + j) + 1] = im;
+      }
+    set_two_target(e, P);
+    return true;
+  }
   if (d.op.kind == F_DIAG && e.op.kind == F_DIAG) return merge_diag(e, d, NR);
   double d2[4], M[8];
   if (d.op.kind == F_DENSE1 && e.op.kind == F_DENSE1 && e.op.pos[0] == d.op.pos[0] && e.op.creg == d.op.creg &&
