@@ -535,14 +535,14 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
   for (int b = 0; b < TB; ++b) {
     const uint64_t bit = (tid >> b) & 1u;
     gin |= bit << H->tbits[H->ts0[b]];
-    gout |= bit << H->tbits[H->ts1[b]];
+    gout |= bit << H->tbo[H->ts1[b]];
   }
   gout ^= H->sflip;
   uint64_t min_[RB], mout[RB];
 #pragma unroll
   for (int b = 0; b < RB; ++b) {
     min_[b] = 1ull << H->tbits[H->rs0[b]];
-    mout[b] = 1ull << H->tbits[H->rs1[b]];
+    mout[b] = 1ull << H->tbo[H->rs1[b]];
   }
   cx<R> a[NR];
   [[maybe_unused]] cx<R> nx[PF ? NR : 1];
@@ -646,7 +646,7 @@ static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, con
   for (size_t i = 0; i < prog.plans.size(); ++i) {
     if (!prog.plans[i].fused) {
       if (prof && (rc = prof_begin(ctx, PROF_GATE, s))) return rc;
-      rc = apply_gate(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &gates[prog.plans[i].single], s);
+      rc = apply_gate(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &prog.pgates[prog.plans[i].single], s);
       if (rc) return rc;
       if (prof && (rc = prof_end(ctx, s))) return rc;
       ++npass;
@@ -677,10 +677,19 @@ static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, con
   return QS_OK;
 }
 
+// Planner settings: LR = 0 (low bits stay on threads at the load and store);
+// QS_FUSION_REMAP=1 lets the planner move qubits between index bits when that
+// takes fewer passes.
+static Config fused_config(int RB, int TB, int L) {
+  Config c{RB, TB, L};
+  if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = rm[0] == '1';
+  return c;
+}
+
 template <typename R, int RB, int TB>
 static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int* passes,
                        hipStream_t s) {
-  const Config c{RB, TB, L};  // LR = 0: low bits stay on threads at the load and store
+  const Config c = fused_config(RB, TB, L);
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;
@@ -722,7 +731,7 @@ template <typename R> __global__ __launch_bounds__(256) void k_basis0(cx<R>* __r
 template <typename R, int RB, int TB>
 static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int init_zero,
                      qs_graph* G) {
-  const Config c{RB, TB, L};
+  const Config c = fused_config(RB, TB, L);
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;

@@ -76,9 +76,9 @@ static FOpW encode_op(const FOp& o) {
 // TB thread-slot LDS offsets of the current layout; the flip term of the
 // current layout; then the RB + TB slot offsets of the new layout.  Offsets
 // address a half-tile image indexed by the tile bits other than h: an offset is
-// the byte offset of the XOR-swizzled position of the single tile bit its slot
-// carries (0 for the slot of h); the swizzle is XOR-linear, so an amplitude's
-// LDS address is the XOR of its slots' offsets.  The write side is carried per
+// the byte offset the single tile bit its slot carries contributes under the
+// relayout's XOR-linear placement (0 for the slot of h), so an amplitude's LDS
+// address is the XOR of its slots' offsets.  The write side is carried per
 // relayout because relabelings change the layout between relayouts.
 struct PassHdr {
   int32_t nops;
@@ -91,22 +91,13 @@ struct PassHdr {
                          // loads: workgroup barrier before the stores (another wave may still have
                          // to load an address this wave stores to)
   int32_t tbits[16];     // tile-local bit -> global bit
+  int32_t tbo[16];       // tile-local bit -> global bit at the store: a permutation of tbits that moves
+                         // qubits between the tile's index bits (the same set of addresses)
   int32_t sorted[16];    // tile bits ascending (global), for the tile base
   int32_t rs0[FMAX_RB], ts0[FMAX_TB];  // load layout (tile-local bits)
   int32_t rs1[FMAX_RB], ts1[FMAX_TB];  // store layout
   uint64_t sflip;        // global-index XOR applied to every store address (pending flips)
 };
-
-// XOR fold of the 3-bit chunks above bit 3: spreads power-of-two strides over
-// LDS banks.  XOR-linear: swz(a ^ b) = swz(a) ^ swz(b).
-__host__ __device__ __forceinline__ uint32_t lds_swz(uint32_t j) {
-  uint32_t h = j >> 3, f = 0;
-  for (int i = 0; i < 5; ++i) {
-    f ^= h & 7;
-    h >>= 3;
-  }
-  return j ^ f;
-}
 
 // ---- host planner --------------------------------------------------------------
 
@@ -116,6 +107,12 @@ struct Config {
   // MI355X (28 qubits, 2000-gate random circuit): 1 or 2 save about 40% of the
   // relayouts but slow every pass by ~20% (half-coalesced tile accesses).
   int LR = 0;
+  // Plan with qubits moved onto the low index bits between passes when that
+  // takes fewer passes.  Off by default: measured on MI355X (30 qubits,
+  // 2000-gate random circuit) it saves 8% of the passes, but they carry a
+  // third more relayouts and a fifth more ops and each takes 10% longer, so
+  // the circuit is 1.5% slower; the noisy 14-qubit density matrix is 4% slower.
+  bool remap = false;
 };
 
 static bool gate_diag_like(const qs_gate& g) { return g.kind == QS_GATE_DIAG || g.kind == QS_GATE_PHASE; }
@@ -200,13 +197,18 @@ struct Layout {
   uint32_t flip = 0;
 };
 
-static Layout make_layout(const std::vector<int>& regs, int K) {
+// Threads take the non-register tile bits in the order `tord` (ascending when
+// empty): the first thread bits are the ones the store writes to the low
+// index bits.
+static Layout make_layout(const std::vector<int>& regs, int K, const std::vector<int>& tord = {}) {
   Layout L;
   L.rs = regs;
   std::vector<char> isreg(K, 0);
   for (int b : regs) isreg[b] = 1;
-  for (int b = 0; b < K; ++b)
+  for (int i = 0; i < K; ++i) {
+    const int b = tord.empty() ? i : tord[i];
     if (!isreg[b]) L.ts.push_back(b);
+  }
   return L;
 }
 
@@ -643,6 +645,7 @@ struct PassPlan {
   int single;                      // gate index when !fused
   std::vector<int> gates;          // gate indices when fused
   std::vector<int> tbits;          // tile-local -> global
+  std::vector<int> tbo;            // tile-local -> global at the store (a permutation of tbits)
 };
 
 // Pass grouping.  A gate may join the current pass when its bits fit in the K
@@ -651,27 +654,54 @@ struct PassPlan {
 // (gates inside the tile cost nothing), so the tile's bits go to the gates
 // that share them most; this packs about a quarter more gates per pass than
 // taking gates in circuit order.
-static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c, std::vector<PassPlan>& plans) {
+//
+// Every tile holds the L lowest index bits (coalesced accesses), so only K - L
+// of its bits are chosen per pass.  With `remap`, each pass's store moves the
+// L qubits of its tile that the next gates need soonest onto the low bits (a
+// permutation of the tile's own index bits, so the same addresses), and the
+// qubits there move to the vacated bits: the next pass then has those qubits
+// for free.  Gates are planned on physical bits (`pg`: the records with their
+// bits translated through the map in force for their pass), and passes without
+// gates at the end put every qubit back on its own bit.
+static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c, bool remap,
+                        std::vector<PassPlan>& plans, std::vector<qs_gate>& pg) {
   const int K = c.RB + c.TB;
+  const int L = c.L;
+  pg.assign(gates, gates + ngates);
   std::vector<int> remaining(ngates);
   for (int i = 0; i < ngates; ++i) remaining[i] = i;
-  const uint64_t low = (1ull << c.L) - 1;
+  const uint64_t low = (1ull << L) - 1;
+  // phys[q]: index bit holding qubit bit q; at[b]: qubit bit held by index bit b
+  std::vector<int> phys(n), at(n);
+  for (int b = 0; b < n; ++b) phys[b] = at[b] = b;
+  auto to_phys = [&](uint64_t m) {
+    uint64_t r = 0;
+    for (; m; m &= m - 1) r |= 1ull << phys[__builtin_ctzll(m)];
+    return r;
+  };
   // gm: every bit of the gate; gx: the bits it mixes (non-diagonal targets; all
   // bits of a gate the tile kernel cannot take).  Two gates commute when every
-  // bit they share is used diagonally by both.
-  std::vector<uint64_t> gm(ngates), gx(ngates);
+  // bit they share is used diagonally by both.  Logical masks (gmq, gxq) are
+  // translated into gm, gx whenever the map changes.
+  std::vector<uint64_t> gmq(ngates), gxq(ngates), gm(ngates), gx(ngates);
   std::vector<int> gcost(ngates);
   std::vector<char> ok(ngates);
   for (int i = 0; i < ngates; ++i) {
     const qs_gate& g = gates[i];
-    gm[i] = gate_mask(g);
+    gmq[i] = gate_mask(g);
     gcost[i] = coef_cost(g, 1 << c.RB);
     ok[i] = fusable(g, c);
-    gx[i] = 0;
-    if (!ok[i]) gx[i] = gm[i];
+    gxq[i] = 0;
+    if (!ok[i]) gxq[i] = gmq[i];
     else if (!gate_diag_like(g))
-      for (int k = 0; k < g.ntargets; ++k) gx[i] |= 1ull << g.bits[g.ncontrols + k];
+      for (int k = 0; k < g.ntargets; ++k) gxq[i] |= 1ull << g.bits[g.ncontrols + k];
+    gm[i] = gmq[i];
+    gx[i] = gxq[i];
   }
+  auto translate = [&](int gi) {
+    qs_gate& g = pg[gi];
+    for (int i = 0; i < g.ncontrols + g.ntargets; ++i) g.bits[i] = phys[gates[gi].bits[i]];
+  };
   std::vector<char> taken(ngates, 0);
   while (!remaining.empty()) {
     PassPlan p;
@@ -681,6 +711,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     if (!ok[remaining[0]]) {
       p.fused = false;
       p.single = remaining[0];
+      translate(p.single);
       remaining.erase(remaining.begin());
       plans.push_back(std::move(p));
       continue;
@@ -715,6 +746,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     }
     // circuit order within the pass (the scheduler reorders commuting gates itself)
     std::sort(p.gates.begin(), p.gates.end());
+    for (int gi : p.gates) translate(gi);
     std::vector<int> rest;
     for (int gi : remaining)
       if (!taken[gi]) rest.push_back(gi);
@@ -724,6 +756,135 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     for (int b = 0; b < n && __builtin_popcountll(S) < K; ++b) S |= 1ull << b;
     for (int b = 0; b < n; ++b)
       if ((S >> b) & 1) p.tbits.push_back(b);  // ascending; the first L are the low bits
+    p.tbo = p.tbits;
+    if (remap && !remaining.empty()) {
+      // the L tile bits whose qubits the next gates (up to 64) use first go low
+      const int kt = (int)p.tbits.size();
+      std::vector<int> first(kt, 1 << 30);
+      int pos = 0;
+      for (int gi : remaining) {
+        if (pos == 64) break;
+        for (int j = 0; j < kt; ++j)
+          if (((gm[gi] >> p.tbits[j]) & 1) && first[j] > pos) first[j] = pos;
+        ++pos;
+      }
+      std::vector<int> order(kt);
+      for (int j = 0; j < kt; ++j) order[j] = j;
+      std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return first[x] < first[y]; });
+      std::vector<char> chosen(kt, 0);
+      for (int i = 0; i < L; ++i) chosen[order[i]] = 1;
+      std::vector<int> out_low, in_high;  // low positions given up, tile bits moving low
+      for (int j = 0; j < L; ++j)
+        if (!chosen[j]) out_low.push_back(j);
+      for (int j = L; j < kt; ++j)
+        if (chosen[j]) in_high.push_back(j);
+      if (!in_high.empty()) {
+        for (size_t i = 0; i < in_high.size(); ++i) {
+          const int jl = out_low[i], jh = in_high[i];
+          const int bl = p.tbits[jl], bh = p.tbits[jh];
+          p.tbo[jl] = bh;
+          p.tbo[jh] = bl;
+          const int ql = at[bl], qh = at[bh];
+          at[bl] = qh;
+          at[bh] = ql;
+          phys[qh] = bl;
+          phys[ql] = bh;
+        }
+        for (int gi : remaining) {
+          gm[gi] = to_phys(gmq[gi]);
+          gx[gi] = to_phys(gxq[gi]);
+        }
+      }
+    }
+    plans.push_back(std::move(p));
+  }
+  if (!remap) return;
+  // Passes without gates that return every qubit to its own bit.  Each moves
+  // the qubits within one tile (the L low bits plus up to K - L others): whole
+  // cycles of the permutation when they fit, else a run along one cycle (all
+  // but its last qubit reach their own bits).
+  for (;;) {
+    std::vector<char> in_tile(n, 0), seen(n, 0);
+    for (int b = 0; b < L; ++b) in_tile[b] = 1;
+    int room = K - L;
+    std::vector<std::vector<int>> cycles;
+    for (int b = 0; b < n; ++b) {
+      if (seen[b] || at[b] == b) continue;
+      std::vector<int> cyc;
+      for (int x = b; !seen[x]; x = at[x]) {  // index bit x's qubit belongs on bit at[x]
+        seen[x] = 1;
+        cyc.push_back(x);
+      }
+      cycles.push_back(cyc);
+    }
+    if (cycles.empty()) break;
+    auto highs = [&](const std::vector<int>& cyc) {
+      int h = 0;
+      for (int x : cyc) h += (x >= L && !in_tile[x]);
+      return h;
+    };
+    std::stable_sort(cycles.begin(), cycles.end(),
+                     [&](const std::vector<int>& u, const std::vector<int>& v) { return highs(u) < highs(v); });
+    bool any = false;
+    for (const auto& cyc : cycles) {
+      const int h = highs(cyc);
+      if (h > room) continue;
+      for (int x : cyc) in_tile[x] = 1;
+      room -= h;
+      any = true;
+    }
+    if (!any) {
+      // a run along the first cycle, starting at one of its low bits if it has one
+      std::vector<int> cyc = cycles[0];
+      for (size_t i = 0; i < cyc.size(); ++i)
+        if (cyc[i] < L) {
+          std::rotate(cyc.begin(), cyc.begin() + i, cyc.end());
+          break;
+        }
+      for (int x : cyc) {
+        if (x >= L && !in_tile[x]) {
+          if (room == 0) break;
+          --room;
+        }
+        in_tile[x] = 1;
+      }
+    }
+    // qubits whose own bit is in the tile go there; the others take the bits left
+    PassPlan p;
+    p.fused = true;
+    p.single = -1;
+    int cnt = 0;
+    for (int b = 0; b < n; ++b) cnt += in_tile[b];
+    for (int b = 0; b < n && cnt < K; ++b)
+      if (!in_tile[b]) {
+        in_tile[b] = 1;
+        ++cnt;
+      }
+    for (int b = 0; b < n; ++b)
+      if (in_tile[b]) p.tbits.push_back(b);
+    const int kt = (int)p.tbits.size();
+    std::vector<int> dest(kt, -1);
+    std::vector<char> used(n, 0);
+    for (int j = 0; j < kt; ++j) {
+      const int q = at[p.tbits[j]];
+      if (in_tile[q]) {
+        dest[j] = q;
+        used[q] = 1;
+      }
+    }
+    int free_b = 0;
+    for (int j = 0; j < kt; ++j) {
+      if (dest[j] >= 0) continue;
+      while (!(in_tile[p.tbits[free_b]] && !used[p.tbits[free_b]])) ++free_b;
+      dest[j] = p.tbits[free_b];
+      used[dest[j]] = 1;
+    }
+    p.tbo = dest;
+    std::vector<int> nat_(n);
+    for (int b = 0; b < n; ++b) nat_[b] = at[b];
+    for (int j = 0; j < kt; ++j) nat_[dest[j]] = at[p.tbits[j]];
+    at.swap(nat_);
+    for (int b = 0; b < n; ++b) phys[at[b]] = b;
     plans.push_back(std::move(p));
   }
 }
@@ -750,6 +911,7 @@ static std::vector<int> choose_regs(const std::vector<int>& need, const std::vec
 // constants, ready for upload.
 struct FusedProgram {
   std::vector<PassPlan> plans;
+  std::vector<qs_gate> pgates;  // the gate records on index bits (single-gate passes run these)
   std::vector<int> hdr_of;  // plan -> header (-1 for single-gate passes)
   std::vector<PassHdr> hdrs;
   std::vector<FOp> ops;
@@ -763,10 +925,48 @@ static uint32_t drop_bit(uint32_t m, int h) { return (m & ((1u << h) - 1)) | ((m
 
 // LDS offsets (bytes) of the slots of layout L in the half-tile image that
 // leaves out tile bit h; the slot carrying h gets offset 0.
-static void slot_offsets_half(const Layout& L, int h, uint32_t amp_bytes, uint32_t* lr, uint32_t* lt) {
-  auto off = [&](int b) -> uint32_t { return b == h ? 0u : lds_swz(drop_bit(1u << b, h)) * amp_bytes; };
-  for (size_t p = 0; p < L.rs.size(); ++p) lr[p] = off(L.rs[p]);
-  for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = off(L.ts[q]);
+// XOR-linear placement of the half-tile image for one relayout from layout A
+// to layout B (tile bit h left out): per tile bit, the amplitude index bits
+// it contributes.  One LDS cycle serves 128 bytes, i.e. the lanes that differ
+// in their first G = log2(128 / amp_bytes) thread bits; those lanes hit
+// distinct banks when their bits' low G index parts are independent.  A's
+// first G thread bits take index bits 0..G-1, every other bit its own index
+// bit above, and B's first G thread bits that are not among A's get the free
+// low bits XORed in.  The images of the K - 1 bits stay independent (each has
+// its own index bit), so the placement is a bijection onto 2^(K-1) entries.
+static std::vector<uint32_t> relayout_placement(const Layout& A, const Layout& B, int h, int K, uint32_t amp_bytes) {
+  int G = 0;
+  while ((amp_bytes << G) < 128u) ++G;
+  std::vector<uint32_t> pl(K, 0);
+  std::vector<char> placed(K, 0);
+  placed[h] = 1;
+  uint32_t low_used = 0;
+  const int ga = std::min(G, (int)A.ts.size());
+  for (int i = 0; i < ga; ++i) {
+    pl[A.ts[i]] = 1u << i;
+    placed[A.ts[i]] = 1;
+  }
+  int hi = ga;
+  for (int b = 0; b < K; ++b)
+    if (!placed[b]) pl[b] = 1u << hi++;
+  const int gb = std::min(ga, (int)B.ts.size());
+  for (int i = 0; i < gb; ++i)
+    if (pl[B.ts[i]] < (1u << ga)) low_used |= pl[B.ts[i]];
+  for (int i = 0; i < gb; ++i) {
+    const int b = B.ts[i];
+    if (pl[b] < (1u << ga)) continue;
+    int k = 0;
+    while ((low_used >> k) & 1) ++k;
+    pl[b] |= 1u << k;
+    low_used |= 1u << k;
+  }
+  return pl;
+}
+
+static void slot_offsets_placed(const Layout& L, const std::vector<uint32_t>& pl, uint32_t amp_bytes, uint32_t* lr,
+                                uint32_t* lt) {
+  for (size_t p = 0; p < L.rs.size(); ++p) lr[p] = pl[L.rs[p]] * amp_bytes;
+  for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = pl[L.ts[q]] * amp_bytes;
 }
 
 static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& c, uint32_t amp_bytes, PassHdr& H,
@@ -774,11 +974,20 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   const int K = c.RB + c.TB;
   std::memset(&H, 0, sizeof(H));
   std::vector<int> g2l(64, -1);
+  // tord: tile bits in the order of the index bits the store writes them to,
+  // for the layouts relayouts build (the store's coalescing and contiguous
+  // runs); the load layout stays ascending
+  std::vector<int> tord;
+  std::vector<char> out_low(K, 0);
   for (int j = 0; j < K; ++j) {
     H.tbits[j] = p.tbits[j];
+    H.tbo[j] = p.tbo[j];
     H.sorted[j] = p.tbits[j];
     g2l[p.tbits[j]] = j;
+    out_low[j] = p.tbo[j] < c.L;
   }
+  for (int j = 0; j < K; ++j) tord.push_back(j);
+  std::sort(tord.begin(), tord.end(), [&](int x, int y) { return p.tbo[x] < p.tbo[y]; });
   // per-gate local bits and lookahead list of needed (non-diagonal target) bits
   std::vector<std::vector<int>> loc(p.gates.size());
   for (size_t k = 0; k < p.gates.size(); ++k) {
@@ -840,14 +1049,20 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
     rl.op.coef = (int)prog.aux.size();
     uint32_t lr[FMAX_RB], lt[FMAX_TB];
     prog.aux.push_back((uint32_t)p);
-    slot_offsets_half(cur, h, amp_bytes, lr, lt);
+    Layout next = make_layout(nr, K, tord);
+    const std::vector<uint32_t> pl = relayout_placement(cur, next, h, K, amp_bytes);
+    slot_offsets_placed(cur, pl, amp_bytes, lr, lt);
     for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
     for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
-    prog.aux.push_back(lds_swz(drop_bit(flip_local(cur), h)) * amp_bytes);
+    uint32_t fo = 0;
+    const uint32_t fl = flip_local(cur);
+    for (int b = 0; b < K; ++b)
+      if (b != h && ((fl >> b) & 1)) fo ^= pl[b];
+    prog.aux.push_back(fo * amp_bytes);
     const uint32_t fp = (cur.flip >> p) & 1u;
-    cur = make_layout(nr, K);
+    cur = next;
     cur.flip = fp << p;
-    slot_offsets_half(cur, h, amp_bytes, lr, lt);
+    slot_offsets_placed(cur, pl, amp_bytes, lr, lt);
     for (int b = 0; b < c.RB; ++b) prog.aux.push_back(lr[b]);
     for (int b = 0; b < c.TB; ++b) prog.aux.push_back(lt[b]);
     built.push_back(rl);
@@ -1013,16 +1228,23 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
     for (int sidx : succ[pick])
       if (--indeg[sidx] == 0) ready.push_back(sidx);
   }
-  // the store layout keeps at most LR of the low L bits in registers
+  // the store layout keeps at most LR of the tile bits stored to the low L
+  // index bits in registers, and the others on the first thread bits
   int low_in_regs = 0;
   for (int b : cur.rs)
-    if (b < c.L) ++low_in_regs;
-  if (low_in_regs > c.LR) {
+    if (out_low[b]) ++low_in_regs;
+  bool low_lanes = true;
+  for (int i = 0; i < c.L - low_in_regs && i < (int)cur.ts.size(); ++i) low_lanes = low_lanes && out_low[cur.ts[i]];
+  if (low_in_regs > c.LR || !low_lanes) {
     std::vector<int> keep;
     int kept_low = 0;
     for (int b : cur.rs)
-      if (b >= c.L || kept_low++ < c.LR) keep.push_back(b);
-    relayout_to(choose_regs(keep, {}, {}, c.RB, K, c.L));
+      if (!out_low[b] || kept_low++ < c.LR) keep.push_back(b);
+    for (int i = K - 1; i >= 0 && (int)keep.size() < c.RB; --i) {
+      const int b = tord[i];
+      if (!out_low[b] && index_in(keep, b) < 0) keep.push_back(b);
+    }
+    relayout_to(keep);
   }
   for (Built& b : built) {
     if (b.op.kind == F_DIAG) {
@@ -1047,8 +1269,9 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   // pending flips go into the store address
   const uint32_t fl = flip_local(cur);
   for (int j = 0; j < K; ++j)
-    if ((fl >> j) & 1) H.sflip |= 1ull << p.tbits[j];
+    if ((fl >> j) & 1) H.sflip |= 1ull << p.tbo[j];
   bool same = H.sflip == 0;
+  for (int j = 0; j < K; ++j) same = same && p.tbo[j] == p.tbits[j];
   for (int b = 0; b < c.RB; ++b) same = same && H.rs1[b] == H.rs0[b];
   for (int b = 0; b < c.TB; ++b) same = same && H.ts1[b] == H.ts0[b];
   H.sync_store = (!H.uses_lds && !same) ? 1 : 0;
@@ -1061,12 +1284,23 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
 
 static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
                          FusedProgram& prog) {
-  plan_groups(gates, ngates, n, c, prog.plans);
+  plan_groups(gates, ngates, n, c, false, prog.plans, prog.pgates);
+  if (c.remap && n >= c.RB + c.TB + 2) {
+    // moving qubits onto the low bits pays when it saves passes, the passes
+    // that put the qubits back included
+    std::vector<PassPlan> plans2;
+    std::vector<qs_gate> pg2;
+    plan_groups(gates, ngates, n, c, true, plans2, pg2);
+    if (plans2.size() < prog.plans.size()) {
+      prog.plans.swap(plans2);
+      prog.pgates.swap(pg2);
+    }
+  }
   prog.hdr_of.assign(prog.plans.size(), -1);
   for (size_t i = 0; i < prog.plans.size(); ++i) {
     if (!prog.plans[i].fused) continue;
     PassHdr H;
-    int rc = schedule_pass(gates, prog.plans[i], c, amp_bytes, H, prog);
+    int rc = schedule_pass(prog.pgates.data(), prog.plans[i], c, amp_bytes, H, prog);
     if (rc) return rc;
     prog.hdr_of[i] = (int)prog.hdrs.size();
     prog.hdrs.push_back(H);

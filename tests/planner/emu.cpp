@@ -5,7 +5,7 @@
 // headers, ops, relayout constants, coefficients) with the semantics of the
 // device interpreter in quict_amd/csrc/qsim_fused.hip: per-tile load with the
 // load layout, each op on the (thread, register) grid, relayouts through a
-// byte-addressed LDS image using the XOR-swizzled slot offsets (every write
+// byte-addressed LDS image using the planned slot offsets (every write
 // checked for range and collisions), lane factors flushed at relayouts and the
 // store, store with the relabeled layout and pending flips.  Single-gate
 // passes and the reference use the C restatement in oracle/csrc/cpu_ref_sim.c.
@@ -46,7 +46,7 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
     for (int b = 0; b < TB; ++b) {
       const uint64_t bit = (tid >> b) & 1u;
       gi |= bit << H.tbits[H.ts0[b]];
-      go |= bit << H.tbits[H.ts1[b]];
+      go |= bit << H.tbo[H.ts1[b]];
     }
     gin[tid] = gi;
     gout[tid] = go ^ H.sflip;
@@ -228,7 +228,7 @@ static void emu_pass(zc* v, int n, int RB, int TB, uint32_t S, const PassHdr& H,
       for (int r = 0; r < NR; ++r) {
         uint64_t g = 0;
         for (int b = 0; b < RB; ++b)
-          if ((r >> b) & 1) g |= 1ull << H.tbits[H.rs1[b]];
+          if ((r >> b) & 1) g |= 1ull << H.tbo[H.rs1[b]];
         v[(tb ^ gout[tid]) ^ g] = a[(size_t)tid * NR + r];
       }
   }
@@ -240,15 +240,18 @@ extern "C" const char* emu_last_error() { return qs::g_err.c_str(); }
 // Plan and run `gates` on the complex128 vector `vec` (2^n entries) with tile
 // shape RB register bits x TB thread bits and L low bits; amp_bytes sets the LDS
 // slot offsets (16 for complex128 tiles, 8 for complex64).  vec == NULL: plan
-// only.  stats (17 ints): passes, fused passes, ops, ops per kind 0..7, largest
+// only.  stats (18 ints): passes, fused passes, ops, ops per kind 0..7, largest
 // pass coefficient count, real dense1, lane diagonals, table diagonals,
-// lane-controlled ops, diagonal + anti-diagonal two-target ops.
+// lane-controlled ops, diagonal + anti-diagonal two-target ops, passes whose
+// store moves qubits between index bits.  QS_FUSION_REMAP=1 lets the planner
+// move qubits between index bits.
 extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, int RB, int TB, int L, int amp_bytes,
                          int* stats) {
   using namespace qs;
   try {
     Config c{RB, TB, L};
     if (const char* lr = getenv("QS_FUSION_LOWREGS")) c.LR = atoi(lr);
+    if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = atoi(rm) == 1;
     FusedProgram prog;
     int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);
     if (rc) return rc;
@@ -260,7 +263,7 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
         continue;
       }
       if (!prog.plans[i].fused) {
-        cpu_apply_gates(vec, n, &gates[prog.plans[i].single], 1, 1);
+        cpu_apply_gates(vec, n, &prog.pgates[prog.plans[i].single], 1, 1);
         continue;
       }
       const PassHdr& H = prog.hdrs[prog.hdr_of[i]];
@@ -296,6 +299,13 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
       stats[14] = dtab;
       stats[15] = lanectl;
       stats[16] = kinds[F_XX2];
+      int moved = 0;  // passes whose store moves qubits between index bits
+      for (auto& h : prog.hdrs) {
+        bool m = false;
+        for (int j = 0; j < RB + TB; ++j) m = m || h.tbo[j] != h.tbits[j];
+        moved += m;
+      }
+      stats[17] = moved;
       (void)ctl1;
       (void)pperm;
     }

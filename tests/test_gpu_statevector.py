@@ -147,6 +147,28 @@ def test_fusion_settings_match_unfused(tile_log2, low_bits, prefetch, precision)
     assert np.max(np.abs(out - ref)) < tol
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_qubits_moved_onto_low_bits_match_unfused(precision, monkeypatch):
+    """Planner option QS_FUSION_REMAP=1: stores that move the qubits needed next
+    onto the low index bits (the same tile addresses, permuted index bits), and
+    passes without gates that put every qubit back; 22 qubits, 1200 gates,
+    where the option takes fewer passes than the default plan."""
+    say("remapped plan 22q", precision)
+    n = 22
+    c = workloads.random_mix_circuit(n, 1200, 37, workloads.RANDOM_GATE_PROBS_60_40)
+    ref = sim(precision, False).run(c).get()
+    plain = sim(precision, True)
+    out0 = plain.run(c).get()
+    monkeypatch.setenv("QS_FUSION_REMAP", "1")
+    moved = sim(precision, True)
+    out1 = moved.run(c).get()
+    print(f"  passes: default {plain.last_passes}, moving qubits {moved.last_passes}")
+    assert moved.last_passes < plain.last_passes
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert np.max(np.abs(out1 - ref)) < tol
+    assert np.max(np.abs(out1 - out0)) < tol
+
+
 def test_relabel_and_flip_passes_are_exact():
     """Uncontrolled X and Swap gates compile to relabelings of the tile layout
     (no arithmetic): the load and store layouts of such passes differ.  Every

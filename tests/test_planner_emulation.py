@@ -53,7 +53,7 @@ def run_both(lib, c, n, shape, L=4, seed=0):
     ref = v.copy()
     lib.cpu_apply_gates(ref.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 1)
     out = v.copy()
-    st = np.zeros(17, dtype=np.int32)
+    st = np.zeros(18, dtype=np.int32)
     RB, TB, S = shape
     rc = lib.emu_apply(out.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), RB, TB, L, S,
                        st.ctypes.data_as(ctypes.c_void_p))
@@ -155,13 +155,41 @@ def test_headline_generator_at_18_qubits(emu):
     assert st[1] < 120  # about twenty gates per pass
 
 
+@pytest.mark.parametrize("shape", [C128, C64], ids=["c128-tile", "c64-tile"])
+@pytest.mark.parametrize("n,ngates,seed", [(19, 500, 33)])
+def test_qubits_moved_onto_low_bits(emu, shape, n, ngates, seed, monkeypatch):
+    """Plans whose stores move the qubits needed next onto the low index bits
+    (and whose last passes put every qubit back; an option, QS_FUSION_REMAP=1)
+    against the reference kernels, and against the plan that moves nothing."""
+    c = workloads.random_mix_circuit(n, ngates, seed, workloads.RANDOM_GATE_PROBS_60_40)
+    monkeypatch.setenv("QS_FUSION_REMAP", "1")
+    out, ref, st = run_both(emu, c, n, shape)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    assert st[17] > 0  # the moving plan was the shorter one here
+    monkeypatch.setenv("QS_FUSION_REMAP", "0")
+    out0, _, st0 = run_both(emu, c, n, shape)
+    assert st0[17] == 0
+    assert st[0] < st0[0]
+    assert np.max(np.abs(out - out0)) < 1e-12
+
+
+@pytest.mark.parametrize("lr", [1])
+def test_qubits_moved_with_low_bits_in_registers(emu, lr, monkeypatch):
+    monkeypatch.setenv("QS_FUSION_LOWREGS", str(lr))
+    monkeypatch.setenv("QS_FUSION_REMAP", "1")
+    c = workloads.random_mix_circuit(20, 900, 41, workloads.RANDOM_GATE_PROBS_60_40)
+    out, ref, st = run_both(emu, c, 20, C128)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    assert st[17] > 0
+
+
 def test_from_zero_state_vs_numpy_oracle(emu):
     n = 14
     c = all_types(n, 250, 9)
     recs, keep = compile_gates(c.gates, n)
     v = np.zeros(1 << n, dtype=np.complex128)
     v[0] = 1
-    st = np.zeros(17, dtype=np.int32)
+    st = np.zeros(18, dtype=np.int32)
     rc = emu.emu_apply(v.ctypes.data_as(ctypes.c_void_p), n, nat.gate_array_ptr(recs), len(recs), 4, 8, 4, 16,
                        st.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
@@ -189,7 +217,7 @@ def test_noise_superoperators_and_xx_ops(emu, shape):
     ref = v.copy()
     emu.cpu_apply_gates(ref.ctypes.data_as(ctypes.c_void_p), 2 * n, nat.gate_array_ptr(recs), len(recs), 1)
     out = v.copy()
-    st = np.zeros(17, dtype=np.int32)
+    st = np.zeros(18, dtype=np.int32)
     RB, TB, S = shape
     rc = emu.emu_apply(out.ctypes.data_as(ctypes.c_void_p), 2 * n, nat.gate_array_ptr(recs), len(recs), RB, TB, 4, S,
                        st.ctypes.data_as(ctypes.c_void_p))
