@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 namespace qs {
@@ -108,10 +109,11 @@ struct Config {
   // relayouts but slow every pass by ~20% (half-coalesced tile accesses).
   int LR = 0;
   // Plan with qubits moved onto the low index bits between passes when that
-  // takes fewer passes.  Off by default: measured on MI355X (30 qubits,
-  // 2000-gate random circuit) it saves 8% of the passes, but they carry a
-  // third more relayouts and a fifth more ops and each takes 10% longer, so
-  // the circuit is 1.5% slower; the noisy 14-qubit density matrix is 4% slower.
+  // takes fewer passes.  Off by default: measured on MI355X, the 30-qubit
+  // 2000-gate random circuit needs 80 instead of 87 passes, but each carries
+  // more gates and takes 6% longer: 632 against 647 ms, within the spread
+  // between machines; the noisy 14-qubit density matrix (26 instead of 29
+  // passes) is 7% slower.
   bool remap = false;
 };
 
@@ -646,6 +648,9 @@ struct PassPlan {
   std::vector<int> gates;          // gate indices when fused
   std::vector<int> tbits;          // tile-local -> global
   std::vector<int> tbo;            // tile-local -> global at the store (a permutation of tbits)
+  int hdr = -1;                    // its header in the program (fused passes)
+  bool move = false;               // the store moves the qubits on the first L lanes onto the low bits
+  std::vector<int> pref;           // move: tile-local bits in the order relayouts give them lanes
 };
 
 // Pass grouping.  A gate may join the current pass when its bits fit in the K
@@ -660,11 +665,17 @@ struct PassPlan {
 // L qubits of its tile that the next gates need soonest onto the low bits (a
 // permutation of the tile's own index bits, so the same addresses), and the
 // qubits there move to the vacated bits: the next pass then has those qubits
-// for free.  Gates are planned on physical bits (`pg`: the records with their
+// for free.  The pass's relayouts give the first lanes to the tile bits whose
+// qubits the next gates use first, and the store moves whatever qubits sit on
+// the first L lanes at the end onto the low bits (so the move itself costs no
+// relayout).  Gates are planned on physical bits (`pg`: the records with their
 // bits translated through the map in force for their pass), and passes without
-// gates at the end put every qubit back on its own bit.
+// gates at the end put every qubit back on its own bit.  Each fused pass goes
+// to `on_pass` (the scheduler) as soon as it is formed: the move is known only
+// after scheduling.
 static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c, bool remap,
-                        std::vector<PassPlan>& plans, std::vector<qs_gate>& pg) {
+                        std::vector<PassPlan>& plans, std::vector<qs_gate>& pg,
+                        const std::function<void(PassPlan&)>& on_pass) {
   const int K = c.RB + c.TB;
   const int L = c.L;
   pg.assign(gates, gates + ngates);
@@ -758,7 +769,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
       if ((S >> b) & 1) p.tbits.push_back(b);  // ascending; the first L are the low bits
     p.tbo = p.tbits;
     if (remap && !remaining.empty()) {
-      // the L tile bits whose qubits the next gates (up to 64) use first go low
+      // lane order: tile bits by the first of the next 64 gates that uses them
       const int kt = (int)p.tbits.size();
       std::vector<int> first(kt, 1 << 30);
       int pos = 0;
@@ -768,28 +779,22 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
           if (((gm[gi] >> p.tbits[j]) & 1) && first[j] > pos) first[j] = pos;
         ++pos;
       }
-      std::vector<int> order(kt);
-      for (int j = 0; j < kt; ++j) order[j] = j;
-      std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return first[x] < first[y]; });
-      std::vector<char> chosen(kt, 0);
-      for (int i = 0; i < L; ++i) chosen[order[i]] = 1;
-      std::vector<int> out_low, in_high;  // low positions given up, tile bits moving low
-      for (int j = 0; j < L; ++j)
-        if (!chosen[j]) out_low.push_back(j);
-      for (int j = L; j < kt; ++j)
-        if (chosen[j]) in_high.push_back(j);
-      if (!in_high.empty()) {
-        for (size_t i = 0; i < in_high.size(); ++i) {
-          const int jl = out_low[i], jh = in_high[i];
-          const int bl = p.tbits[jl], bh = p.tbits[jh];
-          p.tbo[jl] = bh;
-          p.tbo[jh] = bl;
-          const int ql = at[bl], qh = at[bh];
-          at[bl] = qh;
-          at[bh] = ql;
-          phys[qh] = bl;
-          phys[ql] = bh;
-        }
+      p.pref.resize(kt);
+      for (int j = 0; j < kt; ++j) p.pref[j] = j;
+      std::stable_sort(p.pref.begin(), p.pref.end(), [&](int x, int y) { return first[x] < first[y]; });
+      p.move = true;
+    }
+    on_pass(p);
+    if (p.move) {
+      std::vector<int> nat_ = at;
+      bool changed = false;
+      for (size_t j = 0; j < p.tbits.size(); ++j) {
+        nat_[p.tbo[j]] = at[p.tbits[j]];
+        changed = changed || p.tbo[j] != p.tbits[j];
+      }
+      if (changed) {
+        at.swap(nat_);
+        for (int b = 0; b < n; ++b) phys[at[b]] = b;
         for (int gi : remaining) {
           gm[gi] = to_phys(gmq[gi]);
           gx[gi] = to_phys(gxq[gi]);
@@ -885,6 +890,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     for (int j = 0; j < kt; ++j) nat_[dest[j]] = at[p.tbits[j]];
     at.swap(nat_);
     for (int b = 0; b < n; ++b) phys[at[b]] = b;
+    on_pass(p);
     plans.push_back(std::move(p));
   }
 }
@@ -969,7 +975,9 @@ static void slot_offsets_placed(const Layout& L, const std::vector<uint32_t>& pl
   for (size_t q = 0; q < L.ts.size(); ++q) lt[q] = pl[L.ts[q]] * amp_bytes;
 }
 
-static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& c, uint32_t amp_bytes, PassHdr& H,
+// With p.move, relayouts give lanes in the order p.pref and the store layout
+// decides p.tbo: the qubits on the first L lanes go to the low bits.
+static int schedule_pass(const qs_gate* gates, PassPlan& p, const Config& c, uint32_t amp_bytes, PassHdr& H,
                          FusedProgram& prog) {
   const int K = c.RB + c.TB;
   std::memset(&H, 0, sizeof(H));
@@ -986,8 +994,12 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
     g2l[p.tbits[j]] = j;
     out_low[j] = p.tbo[j] < c.L;
   }
-  for (int j = 0; j < K; ++j) tord.push_back(j);
-  std::sort(tord.begin(), tord.end(), [&](int x, int y) { return p.tbo[x] < p.tbo[y]; });
+  if (p.move) {
+    tord = p.pref;
+  } else {
+    for (int j = 0; j < K; ++j) tord.push_back(j);
+    std::sort(tord.begin(), tord.end(), [&](int x, int y) { return p.tbo[x] < p.tbo[y]; });
+  }
   // per-gate local bits and lookahead list of needed (non-diagonal target) bits
   std::vector<std::vector<int>> loc(p.gates.size());
   for (size_t k = 0; k < p.gates.size(); ++k) {
@@ -1235,7 +1247,22 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
     if (out_low[b]) ++low_in_regs;
   bool low_lanes = true;
   for (int i = 0; i < c.L - low_in_regs && i < (int)cur.ts.size(); ++i) low_lanes = low_lanes && out_low[cur.ts[i]];
-  if (low_in_regs > c.LR || !low_lanes) {
+  if (p.move) {
+    // the qubits on the first L lanes go to the low bits; the low bits' qubits
+    // left elsewhere take the index bits those came from
+    std::vector<char> on_low(K, 0);
+    for (int i = 0; i < c.L; ++i) on_low[cur.ts[i]] = 1;
+    std::vector<int> evicted;
+    for (int j = 0; j < c.L; ++j)
+      if (!on_low[j]) evicted.push_back(j);
+    size_t e = 0;
+    for (int i = 0; i < c.L; ++i) {
+      const int m = cur.ts[i];
+      p.tbo[m] = p.tbits[i];
+      if (m >= c.L) p.tbo[evicted[e++]] = p.tbits[m];
+    }
+    for (int j = 0; j < K; ++j) H.tbo[j] = p.tbo[j];
+  } else if (low_in_regs > c.LR || !low_lanes) {
     std::vector<int> keep;
     int kept_low = 0;
     for (int b : cur.rs)
@@ -1282,29 +1309,35 @@ static int schedule_pass(const qs_gate* gates, const PassPlan& p, const Config& 
   return QS_OK;
 }
 
+// Plan and schedule one program (with `remap`: moving qubits onto the low bits).
+static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, bool remap, uint32_t amp_bytes,
+                     FusedProgram& prog) {
+  int rc = QS_OK;
+  auto on_pass = [&](PassPlan& p) {
+    if (rc) return;
+    PassHdr H;
+    rc = schedule_pass(prog.pgates.data(), p, c, amp_bytes, H, prog);
+    if (rc) return;
+    p.hdr = (int)prog.hdrs.size();
+    prog.hdrs.push_back(H);
+  };
+  plan_groups(gates, ngates, n, c, remap, prog.plans, prog.pgates, on_pass);
+  if (rc) return rc;
+  prog.hdr_of.assign(prog.plans.size(), -1);
+  for (size_t i = 0; i < prog.plans.size(); ++i)
+    if (prog.plans[i].fused) prog.hdr_of[i] = prog.plans[i].hdr;
+  return QS_OK;
+}
+
 static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
                          FusedProgram& prog) {
-  plan_groups(gates, ngates, n, c, false, prog.plans, prog.pgates);
-  if (c.remap && n >= c.RB + c.TB + 2) {
-    // moving qubits onto the low bits pays when it saves passes, the passes
-    // that put the qubits back included
-    std::vector<PassPlan> plans2;
-    std::vector<qs_gate> pg2;
-    plan_groups(gates, ngates, n, c, true, plans2, pg2);
-    if (plans2.size() < prog.plans.size()) {
-      prog.plans.swap(plans2);
-      prog.pgates.swap(pg2);
-    }
-  }
-  prog.hdr_of.assign(prog.plans.size(), -1);
-  for (size_t i = 0; i < prog.plans.size(); ++i) {
-    if (!prog.plans[i].fused) continue;
-    PassHdr H;
-    int rc = schedule_pass(prog.pgates.data(), prog.plans[i], c, amp_bytes, H, prog);
-    if (rc) return rc;
-    prog.hdr_of[i] = (int)prog.hdrs.size();
-    prog.hdrs.push_back(H);
-  }
+  int rc = build_one(gates, ngates, n, c, false, amp_bytes, prog);
+  if (rc || !c.remap || n < c.RB + c.TB + 2) return rc;
+  // moving qubits onto the low bits pays when it saves passes, the passes
+  // that put the qubits back included
+  FusedProgram moved;
+  if (build_one(gates, ngates, n, c, true, amp_bytes, moved) == QS_OK && moved.plans.size() < prog.plans.size())
+    prog = std::move(moved);
   return QS_OK;
 }
 
