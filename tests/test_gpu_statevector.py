@@ -157,6 +157,7 @@ def test_qubits_moved_onto_low_bits_match_unfused(precision, monkeypatch):
     n = 22
     c = workloads.random_mix_circuit(n, 1200, 37, workloads.RANDOM_GATE_PROBS_60_40)
     ref = sim(precision, False).run(c).get()
+    monkeypatch.setenv("QS_FUSION_REMAP", "0")
     plain = sim(precision, True)
     out0 = plain.run(c).get()
     monkeypatch.setenv("QS_FUSION_REMAP", "1")
