@@ -30,8 +30,12 @@ enum FKind : int32_t {
 
 constexpr int FMAX_TB = 10;
 constexpr int FMAX_RB = 5;
-// Complex coefficients one pass may carry (staged in LDS next to the relayout buffer).
-constexpr int FMAX_PASS_COEF = 1024;
+// Complex coefficients one pass may carry (staged in LDS next to the relayout
+// buffer).  The planner charges every gate its worst case (a diagonal 4 x 2^RB
+// entries), far above what the passes use after merging (at most 820 entries
+// in a 26-qubit QFT, 288 in the 30-qubit random circuit), so the limit is set
+// where it stops splitting passes; 32 KiB of complex128 in the worst case.
+constexpr int FMAX_PASS_COEF = 2048;
 
 // One op of a pass program, as the host planner builds it.
 struct FOp {
