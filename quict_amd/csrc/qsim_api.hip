@@ -415,7 +415,8 @@ int qs_sv_apply_gates(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
     rc = validate_gate(&gates[i], n);
     if (rc) return rc;
   }
-  if (fuse) return apply_gates_fused(ctx, vec, n, dtype, gates, ngates, passes, as_stream(stream));
+  QS_REQUIRE(fuse >= 0 && fuse <= 2, "fuse must be 0, 1 or 2");
+  if (fuse) return apply_gates_fused(ctx, vec, n, dtype, gates, ngates, fuse == 2, passes, as_stream(stream));
   for (int i = 0; i < ngates; ++i) {
     rc = prof_begin(ctx, PROF_GATE, as_stream(stream));
     if (rc) return rc;

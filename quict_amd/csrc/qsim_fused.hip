@@ -678,18 +678,20 @@ static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, con
 }
 
 // Planner settings: LR = 0 (low bits stay on threads at the load and store);
-// QS_FUSION_REMAP=1 lets the planner move qubits between index bits when that
-// takes fewer passes.
-static Config fused_config(int RB, int TB, int L) {
+// `moves` lets the planner move qubits between index bits when that saves at
+// least 8% of the passes.  QS_FUSION_REMAP overrides it for every call: 0
+// never, 1 whenever it saves a pass.
+static Config fused_config(int RB, int TB, int L, bool moves) {
   Config c{RB, TB, L};
-  if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = rm[0] == '1';
+  c.remap = moves ? 1 : 0;
+  if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = rm[0] == '1' ? 2 : 0;
   return c;
 }
 
 template <typename R, int RB, int TB>
-static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int* passes,
-                       hipStream_t s) {
-  const Config c = fused_config(RB, TB, L);
+static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, bool moves,
+                       int* passes, hipStream_t s) {
+  const Config c = fused_config(RB, TB, L, moves);
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;
@@ -731,7 +733,7 @@ template <typename R> __global__ __launch_bounds__(256) void k_basis0(cx<R>* __r
 template <typename R, int RB, int TB>
 static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int init_zero,
                      qs_graph* G) {
-  const Config c = fused_config(RB, TB, L);
+  const Config c = fused_config(RB, TB, L, false);
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;
@@ -790,8 +792,8 @@ template <typename R, int RB, int TB> static void set_lds_attr() {
   done = true;
 }
 
-int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int* passes,
-                      hipStream_t s) {
+int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, bool moves,
+                      int* passes, hipStream_t s) {
   // Tile shapes (4 register bits per thread, 2^(K-4) threads, a half-tile LDS
   // image for relayouts): K = 12 by default for both precisions (256 threads;
   // 32 KiB of LDS for complex128, 16 KiB for complex64); complex128 also takes
@@ -812,13 +814,13 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
     switch (K) {
       case 11:
         set_lds_attr<double, 4, 7>();
-        return run_fused_t<double, 4, 7>(ctx, v, n, gates, ngates, L, passes, s);
+        return run_fused_t<double, 4, 7>(ctx, v, n, gates, ngates, L, moves, passes, s);
       case 12:
         set_lds_attr<double, 4, 8>();
-        return run_fused_t<double, 4, 8>(ctx, v, n, gates, ngates, L, passes, s);
+        return run_fused_t<double, 4, 8>(ctx, v, n, gates, ngates, L, moves, passes, s);
       case 13:
         set_lds_attr<double, 4, 9>();
-        return run_fused_t<double, 4, 9>(ctx, v, n, gates, ngates, L, passes, s);
+        return run_fused_t<double, 4, 9>(ctx, v, n, gates, ngates, L, moves, passes, s);
       default:
         set_error("complex128 fusion tiles: tile_log2 must be 11, 12 or 13");
         return QS_ERR_UNSUPPORTED;
@@ -828,10 +830,10 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
   switch (K) {
     case 12:
       set_lds_attr<float, 4, 8>();
-      return run_fused_t<float, 4, 8>(ctx, v, n, gates, ngates, L, passes, s);
+      return run_fused_t<float, 4, 8>(ctx, v, n, gates, ngates, L, moves, passes, s);
     case 13:
       set_lds_attr<float, 4, 9>();
-      return run_fused_t<float, 4, 9>(ctx, v, n, gates, ngates, L, passes, s);
+      return run_fused_t<float, 4, 9>(ctx, v, n, gates, ngates, L, moves, passes, s);
     default:
       set_error("complex64 fusion tiles: tile_log2 must be 12 or 13");
       return QS_ERR_UNSUPPORTED;

@@ -112,13 +112,14 @@ struct Config {
   // MI355X (28 qubits, 2000-gate random circuit): 1 or 2 save about 40% of the
   // relayouts but slow every pass by ~20% (half-coalesced tile accesses).
   int LR = 0;
-  // Plan with qubits moved onto the low index bits between passes when that
-  // takes fewer passes.  Off by default: measured on MI355X, the 30-qubit
-  // 2000-gate random circuit needs 80 instead of 87 passes, but each carries
-  // more gates and takes 6% longer: 632 against 647 ms, within the spread
-  // between machines; the noisy 14-qubit density matrix (26 instead of 29
-  // passes) is 7% slower.
-  bool remap = false;
+  // Qubits moved onto the low index bits between passes: 0 never; 1 when that
+  // saves at least 8% of the passes; 2 whenever it saves a pass.  Measured on
+  // MI355X, the 30-qubit 2000-gate random circuit needs 74 instead of 84
+  // passes, each carrying more gates (8.19 against 7.65 ms): 606 against
+  // 643 ms.  With 8% fewer passes the circuit was 2.3% faster; the noisy
+  // density matrix (26 instead of 29 passes) is 5% slower, so its calls do
+  // not ask for it.
+  int remap = 0;
 };
 
 static bool gate_diag_like(const qs_gate& g) { return g.kind == QS_GATE_DIAG || g.kind == QS_GATE_PHASE; }
@@ -1336,12 +1337,15 @@ static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, b
 static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
                          FusedProgram& prog) {
   int rc = build_one(gates, ngates, n, c, false, amp_bytes, prog);
-  if (rc || !c.remap || n < c.RB + c.TB + 2) return rc;
-  // moving qubits onto the low bits pays when it saves passes, the passes
-  // that put the qubits back included
+  if (rc || c.remap == 0 || n < c.RB + c.TB + 2) return rc;
+  // moving qubits onto the low bits pays when it saves enough passes (the
+  // passes that put the qubits back included): the passes get heavier.  Short
+  // programs (under 24 passes) rarely save 8%, and a second plan costs host time.
+  if (c.remap == 1 && prog.plans.size() < 24) return QS_OK;
   FusedProgram moved;
-  if (build_one(gates, ngates, n, c, true, amp_bytes, moved) == QS_OK && moved.plans.size() < prog.plans.size())
-    prog = std::move(moved);
+  if (build_one(gates, ngates, n, c, true, amp_bytes, moved) != QS_OK) return QS_OK;
+  const size_t a = moved.plans.size(), b = prog.plans.size();
+  if (c.remap == 2 ? a < b : 100 * a <= 92 * b) prog = std::move(moved);
   return QS_OK;
 }
 

@@ -23,8 +23,8 @@ int validate_gate(const qs_gate* g, int n);
 int apply_gate(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g, hipStream_t s);
 
 // Fused multi-gate passes (qsim_fused.hip).
-int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int* passes,
-                      hipStream_t s);
+int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, bool moves,
+                      int* passes, hipStream_t s);
 
 // Double-precision reductions (qsim_measure.hip).
 int reduce_prob_zero(qs_ctx* ctx, const void* vec, int n, int dtype, int bit, double* out, hipStream_t s);

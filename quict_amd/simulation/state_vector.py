@@ -203,7 +203,7 @@ class StateVectorSimulator:
             return
         passes = ctypes.c_int(0)
         rc = self._lib.qs_sv_apply_gates(self._ctx, self._vptr(), self._qubits, self._dcode,
-                                         nat.gate_array_ptr(recs), len(recs), 1 if self._fusion else 0,
+                                         nat.gate_array_ptr(recs), len(recs), 2 if self._fusion else 0,
                                          ctypes.byref(passes), self._stream())
         del keep
         nat.check(rc, "qs_sv_apply_gates")

@@ -251,7 +251,7 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
   try {
     Config c{RB, TB, L};
     if (const char* lr = getenv("QS_FUSION_LOWREGS")) c.LR = atoi(lr);
-    if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = atoi(rm) == 1;
+    if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = atoi(rm) == 1 ? 2 : 0;
     FusedProgram prog;
     int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);
     if (rc) return rc;
