@@ -243,14 +243,16 @@ extern "C" const char* emu_last_error() { return qs::g_err.c_str(); }
 // only.  stats (18 ints): passes, fused passes, ops, ops per kind 0..7, largest
 // pass coefficient count, real dense1, lane diagonals, table diagonals,
 // lane-controlled ops, diagonal + anti-diagonal two-target ops, passes whose
-// store moves qubits between index bits.  QS_FUSION_REMAP=1 lets the planner
-// move qubits between index bits.
+// store moves qubits between index bits.  QS_EMU_FUSE=2 plans as
+// qs_sv_apply_gates with fuse=2 (qubit moves when they save at least 8% of the
+// passes); QS_FUSION_REMAP=1 moves qubits whenever that saves a pass.
 extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, int RB, int TB, int L, int amp_bytes,
                          int* stats) {
   using namespace qs;
   try {
     Config c{RB, TB, L};
     if (const char* lr = getenv("QS_FUSION_LOWREGS")) c.LR = atoi(lr);
+    if (const char* fu = getenv("QS_EMU_FUSE")) c.remap = atoi(fu) == 2 ? 1 : 0;  // qs_sv_apply_gates' fuse
     if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = atoi(rm) == 1 ? 2 : 0;
     FusedProgram prog;
     int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);

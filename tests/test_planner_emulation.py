@@ -173,6 +173,26 @@ def test_qubits_moved_onto_low_bits(emu, shape, n, ngates, seed, monkeypatch):
     assert np.max(np.abs(out - out0)) < 1e-12
 
 
+@pytest.mark.parametrize("n,ngates,seed,moves", [(20, 900, 33, True), (21, 1000, 36, False)])
+def test_fuse2_keeps_moves_only_when_they_save_passes(emu, n, ngates, seed, moves, monkeypatch):
+    """The planner mode of qs_sv_apply_gates(fuse=2): the moving plan is kept
+    only when it saves at least 8% of the passes (else the fuse=1 plan runs);
+    either way the result equals the reference kernels."""
+    c = workloads.random_mix_circuit(n, ngates, seed, workloads.RANDOM_GATE_PROBS_60_40)
+    recs, keep = compile_gates(c.gates, n)
+    monkeypatch.setenv("QS_EMU_FUSE", "1")
+    st1 = np.zeros(18, dtype=np.int32)
+    assert emu.emu_apply(None, n, nat.gate_array_ptr(recs), len(recs), 4, 8, 4, 16,
+                         st1.ctypes.data_as(ctypes.c_void_p)) == 0
+    monkeypatch.setenv("QS_EMU_FUSE", "2")
+    out, ref, st2 = run_both(emu, c, n, C128)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    if moves:
+        assert st2[17] > 0 and 100 * st2[0] <= 92 * st1[0]
+    else:
+        assert st2[17] == 0 and st2[0] == st1[0]
+
+
 @pytest.mark.parametrize("lr", [1])
 def test_qubits_moved_with_low_bits_in_registers(emu, lr, monkeypatch):
     monkeypatch.setenv("QS_FUSION_LOWREGS", str(lr))
