@@ -666,14 +666,13 @@ struct PassPlan {
 // taking gates in circuit order.
 //
 // Every tile holds the L lowest index bits (coalesced accesses), so only K - L
-// of its bits are chosen per pass.  With `remap`, each pass's store moves the
-// L qubits of its tile that the next gates need soonest onto the low bits (a
-// permutation of the tile's own index bits, so the same addresses), and the
-// qubits there move to the vacated bits: the next pass then has those qubits
-// for free.  The pass's relayouts give the first lanes to the tile bits whose
-// qubits the next gates use first, and the store moves whatever qubits sit on
-// the first L lanes at the end onto the low bits (so the move itself costs no
-// relayout).  Gates are planned on physical bits (`pg`: the records with their
+// of its bits are chosen per pass.  With `remap`, a pass's store may permute
+// the tile's own index bits (the same addresses): the pass's relayouts give
+// the first lanes to the tile bits whose qubits the next gates use first, and
+// the store moves whatever qubits sit on the first L lanes at the end onto the
+// low bits (so the move itself costs no relayout), the qubits there taking the
+// vacated bits; the next pass then has the moved qubits for free.  Gates are
+// planned on physical bits (`pg`: the records with their
 // bits translated through the map in force for their pass), and passes without
 // gates at the end put every qubit back on its own bit.  Each fused pass goes
 // to `on_pass` (the scheduler) as soon as it is formed: the move is known only
