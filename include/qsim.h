@@ -121,7 +121,7 @@ int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
 int qs_set_fusion_launch(qs_ctx* ctx, int prefetch, int wg_per_cu);
 
 /* Captured gate sequences (HIP graphs).  qs_graph_capture plans the sequence
-   once (the same fused passes as qs_sv_apply_gates with fuse=1), uploads the
+   once (the same fused passes as qs_sv_apply_gates with fuse=2), uploads the
    program into memory the graph owns and captures the passes on `vec` in a HIP
    graph; init_zero != 0 puts a |0...0> initialisation in front.  Every
    qs_graph_launch then replays them on `stream` with no host-side planning:

@@ -733,7 +733,7 @@ template <typename R> __global__ __launch_bounds__(256) void k_basis0(cx<R>* __r
 template <typename R, int RB, int TB>
 static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int init_zero,
                      qs_graph* G) {
-  const Config c = fused_config(RB, TB, L, false);
+  const Config c = fused_config(RB, TB, L, true);  // state vectors: plans as fuse=2
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;

@@ -56,6 +56,32 @@ def test_replays_equal_eager_runs(n, precision):
     cap.close()
 
 
+def test_replay_of_a_plan_with_qubit_moves(monkeypatch):
+    """A circuit long enough for the planner to move qubits onto the low index
+    bits between passes (20 qubits, 900 gates: fewer passes than without
+    moves): replays equal the eager run bit for bit and the per-gate result
+    to 1e-12."""
+    say("graph replay with qubit moves 20q 900 gates")
+    from quict_amd.simulation import StateVectorSimulator
+    n = 20
+    c = workloads.random_mix_circuit(n, 900, 33, workloads.RANDOM_GATE_PROBS_60_40)
+    eager_sim = StateVectorSimulator(precision="double")
+    eager = eager_sim.run(c).get()
+    unfused = StateVectorSimulator(precision="double", fusion=False).run(c).get()
+    assert np.max(np.abs(eager - unfused)) < 1e-12
+    sim = StateVectorSimulator(precision="double")
+    cap = sim.capture(c)
+    assert cap.passes == eager_sim.last_passes
+    for k in range(2):
+        assert np.array_equal(cap.replay().get(), eager), f"replay {k}"
+    cap.close()
+    monkeypatch.setenv("QS_FUSION_REMAP", "0")
+    plain = StateVectorSimulator(precision="double")
+    plain.run(c)
+    print(f"  passes with qubit moves {eager_sim.last_passes}, without {plain.last_passes}")
+    assert eager_sim.last_passes < plain.last_passes
+
+
 def test_capture_without_init_applies_to_the_buffer():
     say("graph no init")
     import torch
