@@ -112,7 +112,8 @@ int qs_sv_apply_gates(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 
 /* Fusion tile shape: log2 amplitudes per tile (0 = built-in default 12; also
    11 or 13 for complex128, 13 for complex64) and the number of low bits always
-   kept contiguous (0 = 4, or 2..8). */
+   kept contiguous (0 = 4, or 2..8, at most tile_log2 - 4; checked again when
+   a fused call runs, since the two may be set in either order). */
 int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits);
 
 /* Launch shape of the fused tile pass: prefetch != 0 selects persistent
@@ -128,8 +129,10 @@ int qs_set_fusion_launch(qs_ctx* ctx, int prefetch, int wg_per_cu);
    repeated runs of one circuit (sampling loops, serving) cost only the GPU
    passes.  The graph is bound to `vec` (same buffer, size and dtype) until
    qs_graph_destroy.  Sequences with a gate the tile kernel cannot take (dense
-   on more than 4 targets) and registers smaller than 2^(tile+1) amplitudes are
-   not captured (QS_ERR_UNSUPPORTED).  Replaces the per-run gate loop of
+   on more than 4 targets, diagonal or phase on more than 6 bits, more than
+   tile_log2 bits, or more than tile_log2 - low_bits bits above the low bits)
+   and registers smaller than 2^(tile+1) amplitudes are not captured
+   (QS_ERR_UNSUPPORTED).  Replaces the per-run gate loop of
    StateVectorSimulator.run (statevector_simulator.py:131-166) for a fixed
    circuit; the reference has no captured form. */
 typedef struct qs_graph qs_graph;

@@ -351,6 +351,8 @@ int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits) {
   QS_REQUIRE(ctx, "null ctx");
   QS_REQUIRE(tile_log2 == 0 || (tile_log2 >= 11 && tile_log2 <= 13), "tile_log2 must be 0 or in [11, 13]");
   QS_REQUIRE(low_bits == 0 || (low_bits >= 2 && low_bits <= 8), "low_bits must be 0 or in [2, 8]");
+  // a tile of 2^K amplitudes keeps its low bits on its K - 4 thread slots
+  QS_REQUIRE(low_bits <= (tile_log2 ? tile_log2 : 12) - 4, "low_bits must not exceed tile_log2 - 4");
   ctx->tile_log2 = tile_log2;
   ctx->low_bits = low_bits;
   return QS_OK;

@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 
 namespace qs {
 
@@ -739,8 +740,10 @@ static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int nga
   if (rc) return rc;
   for (const PassPlan& p : prog.plans)
     if (!p.fused) {
-      set_error("qs_graph_capture: a gate the tile kernel cannot take (more than 4 register targets) "
-                "would run as its own pass; such sequences are not captured");
+      set_error("qs_graph_capture: a gate the tile kernel cannot take would run as its own per-gate pass "
+                "(a dense gate on more than 4 targets, a diagonal or phase gate on more than 6 bits, or a gate "
+                "with more than tile_log2 bits, or more than tile_log2 - low_bits bits above the low bits); "
+                "such sequences are not captured");
       return QS_ERR_UNSUPPORTED;
     }
   ProgImage im;
@@ -792,14 +795,32 @@ template <typename R, int RB, int TB> static void set_lds_attr() {
   done = true;
 }
 
+static int apply_gates_fused_impl(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
+                                  bool moves, int* passes, hipStream_t s);
+
+// The planner is host C++: no exception may cross the C ABI.
 int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, bool moves,
                       int* passes, hipStream_t s) {
+  try {
+    return apply_gates_fused_impl(ctx, vec, n, dtype, gates, ngates, moves, passes, s);
+  } catch (const std::exception& e) {
+    set_error(std::string("fusion planner: ") + e.what());
+    return QS_ERR_DEVICE;
+  }
+}
+
+static int apply_gates_fused_impl(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
+                                  bool moves, int* passes, hipStream_t s) {
   // Tile shapes (4 register bits per thread, 2^(K-4) threads, a half-tile LDS
   // image for relayouts): K = 12 by default for both precisions (256 threads;
   // 32 KiB of LDS for complex128, 16 KiB for complex64); complex128 also takes
   // 11 or 13, complex64 13.
   const int K = ctx->tile_log2 ? ctx->tile_log2 : 12;
   const int L = ctx->low_bits ? ctx->low_bits : 4;
+  if (L > K - 4) {  // tile shape and low bits may be set in either order
+    set_error("low_bits must not exceed tile_log2 - 4");
+    return QS_ERR_ARG;
+  }
   if (n < K + 1 || ngates == 0) {
     // small registers: every pass is latency-bound anyway; one kernel per gate
     for (int i = 0; i < ngates; ++i) {
@@ -844,10 +865,27 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 
 namespace qs {
 
+static int graph_capture_impl(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
+                              int init_zero, qs_graph* G);
+
 static int graph_capture(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates, int init_zero,
                          qs_graph* G) {
+  try {
+    return graph_capture_impl(ctx, vec, n, dtype, gates, ngates, init_zero, G);
+  } catch (const std::exception& e) {
+    set_error(std::string("fusion planner: ") + e.what());
+    return QS_ERR_DEVICE;
+  }
+}
+
+static int graph_capture_impl(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* gates, int ngates,
+                              int init_zero, qs_graph* G) {
   const int K = ctx->tile_log2 ? ctx->tile_log2 : 12;
   const int L = ctx->low_bits ? ctx->low_bits : 4;
+  if (L > K - 4) {
+    set_error("low_bits must not exceed tile_log2 - 4");
+    return QS_ERR_ARG;
+  }
   if (n < K + 1) {
     set_error("qs_graph_capture: registers below 2^(tile+1) amplitudes run one kernel per gate and are not captured");
     return QS_ERR_UNSUPPORTED;

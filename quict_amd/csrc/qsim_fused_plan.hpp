@@ -759,6 +759,17 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
       coef_budget -= gcost[best];
       p.gates.push_back(best);
     }
+    if (p.gates.empty()) {
+      // No gate fits beside the L low bits (the head gate has more than K - L
+      // bits above them, e.g. a 10-bit controlled gate on high qubits): the
+      // head gate, which nothing precedes, runs as its own per-gate pass.
+      p.fused = false;
+      p.single = remaining[0];
+      translate(p.single);
+      remaining.erase(remaining.begin());
+      plans.push_back(std::move(p));
+      continue;
+    }
     // circuit order within the pass (the scheduler reorders commuting gates itself)
     std::sort(p.gates.begin(), p.gates.end());
     for (int gi : p.gates) translate(gi);
@@ -1335,6 +1346,11 @@ static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, b
 
 static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
                          FusedProgram& prog) {
+  // every tile keeps its L low bits on thread slots at the load and the store
+  if (c.L < 1 || c.L > c.TB) {
+    set_error("fusion planner: low_bits must be in [1, tile_log2 - 4]");
+    return QS_ERR_ARG;
+  }
   int rc = build_one(gates, ngates, n, c, false, amp_bytes, prog);
   if (rc || c.remap == 0 || n < c.RB + c.TB + 2) return rc;
   // moving qubits onto the low bits pays when it saves enough passes (the

@@ -62,6 +62,7 @@ def test_replay_of_a_plan_with_qubit_moves(monkeypatch):
     moves): replays equal the eager run bit for bit and the per-gate result
     to 1e-12."""
     say("graph replay with qubit moves 20q 900 gates")
+    monkeypatch.delenv("QS_FUSION_REMAP", raising=False)  # the default fuse=2 rule plans both runs
     from quict_amd.simulation import StateVectorSimulator
     n = 20
     c = workloads.random_mix_circuit(n, 900, 33, workloads.RANDOM_GATE_PROBS_60_40)

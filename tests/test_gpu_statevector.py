@@ -147,6 +147,39 @@ def test_fusion_settings_match_unfused(tile_log2, low_bits, prefetch, precision)
     assert np.max(np.abs(out - ref)) < tol
 
 
+def test_low_bits_beyond_the_thread_slots_are_refused():
+    """An 11-bit tile has 7 thread slots: qs_set_fusion refuses 8 low bits
+    before anything runs; 7 are accepted."""
+    say("qs_set_fusion refuses low_bits > tile_log2 - 4")
+    from quict_amd import _native as nat
+    lib = nat.load_library()
+    ctx = nat.context(0)
+    try:
+        assert lib.qs_set_fusion(ctx, 11, 8) != 0
+        assert b"low_bits" in lib.qs_last_error()
+        nat.check(lib.qs_set_fusion(ctx, 11, 7))
+    finally:
+        lib.qs_set_fusion(ctx, 0, 0)
+
+
+def test_wide_controlled_gate_between_fused_passes():
+    """A 9-controlled X (10-qubit Unitary on qubits 0..9) inside a fused
+    20-qubit circuit: it has more bits above the low bits than a tile holds,
+    so it runs as its own per-gate pass; fused equals per-gate to 1e-12."""
+    say("wide controlled gate in a fused circuit 20q")
+    from quict_amd.core import Unitary
+    n = 20
+    m = np.eye(1 << 10, dtype=np.complex128)
+    m[-2:, -2:] = [[0, 1], [1, 0]]
+    c = workloads.random_mix_circuit(n, 200, 41, workloads.RANDOM_GATE_PROBS_60_40)
+    Unitary(m) | c(list(range(10)))
+    for g in workloads.random_mix_circuit(n, 200, 42, workloads.RANDOM_GATE_PROBS_60_40).gates:
+        c.append(g)
+    ref = sim("double", False).run(c).get()
+    out = sim("double", True).run(c).get()
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_qubits_moved_onto_low_bits_match_unfused(precision, monkeypatch):
     """Planner option QS_FUSION_REMAP=1: stores that move the qubits needed next

@@ -148,6 +148,39 @@ def test_low_bits_in_registers_at_load_and_store(emu, lr, monkeypatch):
     assert np.max(np.abs(out - ref)) < 1e-12
 
 
+def test_low_bits_up_to_the_thread_bits(emu):
+    """An 11-bit tile has 7 thread slots: 7 low bits plan and run, 8 are refused
+    (the planner keeps the low bits on thread slots at the load and store)."""
+    c = workloads.random_mix_circuit(16, 300, 21, workloads.RANDOM_GATE_PROBS_60_40)
+    out, ref, st = run_both(emu, c, 16, (4, 7, 16), L=7)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    recs, _ = compile_gates(c.gates, 16)
+    v = np.zeros(1 << 16, dtype=np.complex128)
+    st = np.zeros(18, dtype=np.int32)
+    rc = emu.emu_apply(v.ctypes.data_as(ctypes.c_void_p), 16, nat.gate_array_ptr(recs), len(recs), 4, 7, 8, 16,
+                       st.ctypes.data_as(ctypes.c_void_p))
+    assert rc != 0 and b"low_bits" in emu.emu_last_error()
+
+
+@pytest.mark.parametrize("remap", ["0", "1"])
+def test_wide_controlled_gate_on_high_bits(emu, remap, monkeypatch):
+    """A 9-controlled X as a 10-qubit Unitary on qubits 0..9 (index bits 15..6):
+    more bits above the 4 low bits than a 12-bit tile has room for, so it runs
+    as its own per-gate pass between fused passes."""
+    monkeypatch.setenv("QS_FUSION_REMAP", remap)
+    n = 16
+    m = np.eye(1 << 10, dtype=np.complex128)
+    m[-2:, -2:] = [[0, 1], [1, 0]]
+    c = workloads.random_mix_circuit(n, 150, 41, workloads.RANDOM_GATE_PROBS_60_40)
+    Unitary(m) | c(list(range(10)))
+    c2 = workloads.random_mix_circuit(n, 150, 42, workloads.RANDOM_GATE_PROBS_60_40)
+    for g in c2.gates:
+        c.append(g)
+    out, ref, st = run_both(emu, c, n, C128)
+    assert st[0] > st[1]  # one per-gate pass besides the fused ones
+    assert np.max(np.abs(out - ref)) < 1e-12
+
+
 def test_headline_generator_at_18_qubits(emu):
     """The 2000-gate 60/40 generator of the 30-qubit workload, at 18 qubits."""
     out, ref, st = run_both(emu, workloads.sv30_circuit(n=18), 18, C128)
