@@ -325,6 +325,61 @@ def run_unitary14(lib, ctx, reps=3, cpu=True):
     return out
 
 
+def run_dense30(lib, ctx, n=30, reps=3):
+    """Dense gates on 5 and 6 targets (a random Unitary and a QFT block) on a
+    30-qubit complex128 state, one gate per pass: the matrix-core kernel
+    (v_mfma_f64_16x16x4) against the per-amplitude kernel it replaces
+    (QS_DENSE_MFMA=0), each timed with HIP events on the launch stream."""
+    import numpy as np
+    import torch
+    from quict_amd.core import Circuit, QFT, Unitary
+    from quict_amd.simulation.gate_compiler import compile_gates
+    v = torch.zeros(1 << n, dtype=torch.complex128, device="cuda:0")
+    v[0] = 1
+    stream = nat_stream()
+    out = {"qubits": n}
+    rng = np.random.RandomState(3)
+    for k, args in [(5, [3, 17, 9, 25, 0]), (6, [28, 4, 13, 21, 7, 1])]:
+        for kind in ("unitary", "qft"):
+            c = Circuit(n)
+            if kind == "unitary":
+                q, _ = np.linalg.qr(rng.randn(1 << k, 1 << k) + 1j * rng.randn(1 << k, 1 << k))
+                Unitary(q) | c(args)
+            else:
+                QFT(k) | c(args)
+            recs, keep = compile_gates(c.gates, n)
+            for mode in ("mfma", "per_amplitude"):
+                os.environ["QS_DENSE_MFMA"] = "1" if mode == "mfma" else "0"
+                from quict_amd import _native as nat
+                call = lambda: nat.check(lib.qs_sv_apply_gates(ctx, ctypes.c_void_p(v.data_ptr()), n, nat.QS_C128,
+                                                                nat.gate_array_ptr(recs), len(recs), 0, None, stream),
+                                         "qs_sv_apply_gates")
+                call()
+                torch.cuda.synchronize()
+                profile_read(lib, ctx, reset=True)
+                lib.qs_profile_enable(ctx, 1)
+                for _ in range(reps):
+                    call()
+                torch.cuda.synchronize()
+                lib.qs_profile_enable(ctx, 0)
+                ms, cnt = profile_read(lib, ctx, reset=True)
+                avg = ms[1] / max(cnt[1], 1)
+                flops = 8.0 * (1 << k) * (1 << n)
+                out[f"{kind}{k}_{mode}"] = {"avg_ms": avg, "touched_gbs": 2 * (1 << n) * 16 / (avg * 1e-3) / 1e9,
+                                            "tflops": flops / (avg * 1e-3) / 1e12,
+                                            "frac_fp64_peak": flops / (avg * 1e-3) / 78.6e12}
+    os.environ.pop("QS_DENSE_MFMA", None)
+    out["norm_after"] = float(torch.linalg.vector_norm(v).item())
+    del v
+    torch.cuda.empty_cache()
+    return out
+
+
+def nat_stream():
+    from quict_amd import _native as nat
+    return nat.stream_ptr(0)
+
+
 def measured_traffic(n, ngates):
     """HBM bytes per tile-pass launch from the committed PMC summary (FETCH_SIZE +
     WRITE_SIZE, copy-calibrated) when this run is the workload it was measured on."""
@@ -391,9 +446,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--gates", type=int, default=2000)
+    ap.add_argument("--low-bits", type=int, default=0,
+                    help="contiguous low index bits per fused tile (qs_set_fusion; 0 = built-in default 4)")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["dm14", "unitary14", "adjoint20", "qnn_c128", "qnn_c64"],
+    ap.add_argument("--only", choices=["dm14", "unitary14", "adjoint20", "qnn_c128", "qnn_c64", "dense30"],
                     help="run one secondary workload alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -406,13 +463,16 @@ def main():
 
     lib = nat.load_library()
     ctx = nat.context(0)
+    if args.low_bits:
+        nat.check(lib.qs_set_fusion(ctx, 0, args.low_bits), "qs_set_fusion")
     if args.only:
         log(f"secondary workload {args.only} alone")
         only = {"dm14": lambda: run_dm14(lib, ctx, cpu=not args.no_cpu),
                 "unitary14": lambda: run_unitary14(lib, ctx, cpu=not args.no_cpu),
                 "adjoint20": lambda: run_adjoint_20q(cpu=not args.no_cpu),
                 "qnn_c128": lambda: run_qnn_batch("double"),
-                "qnn_c64": lambda: run_qnn_batch("single")}[args.only]()
+                "qnn_c64": lambda: run_qnn_batch("single"),
+                "dense30": lambda: run_dense30(lib, ctx)}[args.only]()
         print(json.dumps({"workload": args.only, **only}))
         return
     n = args.qubits
@@ -515,7 +575,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"state-vector {n}q complex128, seeded random {args.gates}-gate circuit "
                                "(60% 1q / 40% 2q, seed 2), fused tile passes",
-                   "qubits": n, "gates": args.gates, "seed": 2, "state_gib": state_bytes / 2 ** 30},
+                   "qubits": n, "gates": args.gates, "seed": 2, "state_gib": state_bytes / 2 ** 30,
+                   "tile_low_bits": args.low_bits or 4},
         "gates_per_s": gates_per_s,
         "passes_per_step": passes,
         "roofline": {"bound": "hbm", "achieved": roof_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

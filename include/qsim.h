@@ -258,7 +258,9 @@ typedef struct qs_qnn_config {
    shared: theta[(l*2 + g)*max_width + q].  Loss: mean over circuits of
    (<Z_0> - target[c])^2.  Outputs (host): expval[c], loss, grad_theta
    (d loss / d theta), grad_x (per circuit, nfeat; may be NULL).  dtype selects
-   complex64 / complex128 amplitudes (reductions in double).  cfg may be NULL. */
+   complex64 / complex128 amplitudes (reductions in double).  cfg may be NULL.
+   grad_theta is the sum of per-circuit terms taken in circuit order, so it is
+   bit-identical across runs and launch shapes of the same block_threads. */
 int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int nfeat, const double* x,
                  const double* target, int layers, int max_width, const double* theta, double* expval,
                  double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg, void* stream);

@@ -223,6 +223,8 @@ template <typename R> int apply_gate_t(qs_ctx* ctx, cx<R>* v, int n, const qs_ga
     case QS_GATE_DENSE:
       if (t == 1) return launch_kind<R, QS_GATE_DENSE, 1>(v, n, g, s);
       if (t == 2) return launch_kind<R, QS_GATE_DENSE, 2>(v, n, g, s);
+      if (dense_mfma_eligible(g, n))
+        return launch_dense_mfma(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, g, s);
       return launch_dense_k<R>(ctx, v, n, g, s);
     case QS_GATE_DIAG:
       if (t == 1) return launch_kind<R, QS_GATE_DIAG, 1>(v, n, g, s);

@@ -29,6 +29,10 @@ int apply_gates_fused(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 // Double-precision reductions (qsim_measure.hip).
 int reduce_prob_zero(qs_ctx* ctx, const void* vec, int n, int dtype, int bit, double* out, hipStream_t s);
 
+// Dense gates on 4..6 targets on the matrix cores (qsim_dense.hip).
+bool dense_mfma_eligible(const qs_gate* g, int n);
+int launch_dense_mfma(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g, hipStream_t s);
+
 inline size_t amp_bytes(int dtype) { return dtype == QS_C128 ? 16 : 8; }
 
 }  // namespace qs

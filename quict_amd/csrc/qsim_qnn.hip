@@ -45,7 +45,7 @@ struct QnnArgs {
   const double* theta;     // layers x 2 x max_width
   double* expval;          // ncirc
   double* loss_terms;      // ncirc
-  double* grad_theta;      // layers x 2 x max_width (accumulated with atomics)
+  double* grad_circ;       // ncirc x (layers x 2 x max_width): each circuit's term of d(loss)/d(theta)
   double* grad_x;          // ncirc x nfeat
   void* slots;             // global per-workgroup state slots (2 x 2^max_width amplitudes each)
   int* counter;
@@ -258,17 +258,28 @@ __global__ __launch_bounds__(1024) void k_qnn(QnnArgs A) {
       A.expval[c] = e;
       A.loss_terms[c] = (e - tgt) * (e - tgt);
     }
+    // this circuit's row of the per-circuit gradient terms (summed in circuit
+    // order by k_qnn_reduce: the result does not depend on which workgroup ran
+    // which circuit)
     for (int i = tid; i < A.layers * 2 * MW; i += nt) {
       const int q = i % MW;
-      if (q < w && gacc[i] != 0.0) atomicAdd(A.grad_theta + i, dl * gacc[i]);
+      A.grad_circ[(uint64_t)c * A.layers * 2 * MW + i] = q < w ? dl * gacc[i] : 0.0;
     }
     __syncthreads();
   }
 }
 
-__global__ void k_qnn_clear(double* g, int ng, int* counter) {
-  for (int i = threadIdx.x; i < ng; i += blockDim.x) g[i] = 0;
+__global__ void k_qnn_clear(int* counter) {
   if (threadIdx.x == 0) *counter = 0;
+}
+
+// grad_theta[i] = sum over circuits c = 0, 1, ... (in that order) of grad_circ[c][i]
+__global__ void k_qnn_reduce(const double* __restrict__ grad_circ, int ncirc, int nth, double* __restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nth) return;
+  double s = 0;
+  for (int c = 0; c < ncirc; ++c) s += grad_circ[(uint64_t)c * nth + i];
+  g[i] = s;
 }
 
 struct QnnPlan {
@@ -299,13 +310,15 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   int dev = ctx->device, cus = 0;
   QS_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int blocks = std::max(1, cus * per_cu);
-  // device buffers: order, widths, x, target, theta, expval, loss, grad_theta, grad_x, counter, slots
+  // device buffers: order, widths, x, target, theta, expval, loss, grad_theta, grad_x, counter,
+  // per-circuit gradient terms, slots
   const uint64_t slot_bytes = (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>);
   auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
   const uint64_t nth = (uint64_t)layers * 2 * max_width;
   const uint64_t sz_i = al(ncirc * sizeof(int32_t)), sz_x = al((uint64_t)ncirc * nfeat * sizeof(double)),
                  sz_c = al(ncirc * sizeof(double)), sz_t = al(nth * sizeof(double));
-  const uint64_t total = 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + slot_bytes;
+  const uint64_t sz_gc = al((uint64_t)ncirc * nth * sizeof(double));
+  const uint64_t total = 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + sz_gc + slot_bytes;
   QnnPlan& P = g_plan;
   const bool same = P.exec && P.dtype == dtype && P.ncirc == ncirc && P.nfeat == nfeat && P.layers == layers &&
                     P.max_width == max_width && P.lds_max_width == lds_w && P.threads == threads &&
@@ -334,7 +347,8 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   double* d_th = (double*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c);
   double* d_gth = (double*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c + sz_t);
   int* d_cnt = (int*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t);
-  void* d_slots = b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256;
+  double* d_gc = (double*)(b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256);
+  void* d_slots = b + 2 * sz_i + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + sz_gc;
 
   // widest first
   std::vector<int32_t> order(ncirc);
@@ -360,7 +374,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   A.theta = d_th;
   A.expval = d_e;
   A.loss_terms = d_l;
-  A.grad_theta = d_gth;
+  A.grad_circ = d_gc;
   A.grad_x = d_gx;
   A.slots = d_slots;
   A.counter = d_cnt;
@@ -372,18 +386,24 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   if (rc) return rc;
   if (use_graph) {
     if (!same) {
-      // clear + persistent kernel, captured once and replayed for every batch of this shape
+      // queue reset + persistent kernel + ordered reduction, captured once and
+      // replayed for every batch of this shape
       QS_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(256), 0, s, d_gth, (int)nth, d_cnt);
+      hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(64), 0, s, d_cnt);
       hipLaunchKernelGGL(k_qnn<R>, dim3(blocks), dim3(threads), lds_bytes, s, A);
+      hipLaunchKernelGGL(k_qnn_reduce, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, d_gc, ncirc, (int)nth,
+                         d_gth);
       QS_CHECK_HIP(hipStreamEndCapture(s, &P.graph));
       QS_CHECK_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
     }
     QS_CHECK_HIP(hipGraphLaunch(P.exec, s));
   } else {
-    hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(256), 0, s, d_gth, (int)nth, d_cnt);
+    hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(64), 0, s, d_cnt);
     QS_CHECK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_qnn<R>, dim3(blocks), dim3(threads), lds_bytes, s, A);
+    QS_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_qnn_reduce, dim3((unsigned)((nth + 255) / 256)), dim3(256), 0, s, d_gc, ncirc, (int)nth,
+                       d_gth);
     QS_CHECK_HIP(hipGetLastError());
   }
   rc = prof_end(ctx, s);

@@ -40,6 +40,24 @@ def test_hea_adjoint_vs_oracle_shift(n, layers, seed):
     assert np.max(np.abs(g - ref)) < 1e-10
 
 
+def test_hea_14q_adjoint_vs_oracle_shift_subset():
+    """14 qubits, 4 layers (112 parameters): the GPU adjoint gradient against the
+    CPU oracle's parameter shift on 10 parameters spread over the layers."""
+    say("hea adjoint 14q vs cpu shift subset")
+    from quict_amd.gradient.adjoint import ParameterizedCircuit, PauliSum, adjoint_gradient
+    n, layers = 14, 4
+    pc = ParameterizedCircuit.hea(n, layers)
+    th = workloads.hea_params(n, layers, 6).ravel()
+    terms = workloads.zz_x_hamiltonian(n)
+    E, g = adjoint_gradient(pc, th, PauliSum(n, terms))
+    assert abs(E - og.expectation(pc, th, terms)) < 1e-12
+    sub = [0, 13, 14, 27, 40, 55, 61, 80, 97, 111]
+    ref = og.parameter_shift(pc, th, terms, params=sub)
+    err = float(np.max(np.abs(g[sub] - ref[sub])))
+    print(f"  max |adjoint - oracle shift| on 10 parameters: {err:.2e}")
+    assert err < 1e-10
+
+
 def test_mixed_generators_adjoint():
     say("mixed generators")
     from quict_amd.gradient.adjoint import ParameterizedCircuit, PauliSum, adjoint_gradient

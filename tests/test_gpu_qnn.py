@@ -106,3 +106,56 @@ def test_config_batch_sample_vs_gpu_adjoint(precision):
     assert err_e < tol_e and err_g < tol_g
     assert np.max(np.abs(ev[idx] - sev)) < (1e-12 if precision == "double" else 1e-5)
     assert np.isfinite(loss) and np.all(np.isfinite(gth))
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_config_batch_widths_8_to_14_vs_cpu_oracle(precision):
+    """Circuits of the 1024-circuit configuration batch at widths 8..14: each
+    one's <Z_0> from the full batch against the CPU oracle, and its batched
+    gradient (as a one-circuit batch, loss (E - t)^2) against
+    2 (E - t) * the oracle's parameter shift on four parameters.  complex128 at
+    1e-12 / 1e-10; the complex64 error against the same oracle is printed."""
+    say("qnn config widths 8..14 vs cpu oracle", precision)
+    from quict_amd.gradient.qnn import QNNBatch, synthetic_batch, circuit_for
+    widths, x, t = synthetic_batch(1024, 16, 8, 18, 4)
+    layers, mw = 6, 18
+    theta = np.random.RandomState(7).uniform(-np.pi, np.pi, layers * 2 * mw)
+    qb = QNNBatch(widths, x, t, layers=layers, max_width=mw, precision=precision)
+    _, ev, _, _ = qb.forward_backward(theta)
+    err_e = err_g = 0.0
+    for w in range(8, 15):
+        c = int(np.where(widths == w)[0][0])
+        pc = circuit_for(w, x[c], layers, mw)
+        z = [(1.0, "Z" + "I" * (w - 1))]
+        e_ref = og.expectation(pc, theta, z)
+        err_e = max(err_e, abs(ev[c] - e_ref))
+        params = [(0 * 2 + 0) * mw + 0, (2 * 2 + 1) * mw + w // 2, (4 * 2 + 0) * mw + w - 1, (5 * 2 + 1) * mw + 1]
+        ps = og.parameter_shift(pc, theta, z, params=params)
+        one = QNNBatch(widths[c:c + 1], x[c:c + 1], t[c:c + 1], layers=layers, max_width=mw, precision=precision)
+        _, ev1, g1, _ = one.forward_backward(theta)
+        err_g = max(err_g, float(np.max(np.abs(g1[params] - 2 * (e_ref - t[c]) * ps[params]))))
+    print(f"  {precision}: max |E - oracle| = {err_e:.2e}, max |grad - oracle shift| = {err_g:.2e}")
+    if precision == "double":
+        assert err_e < 1e-12 and err_g < 1e-10
+    else:
+        assert err_e < 1e-4 and err_g < 1e-4
+
+
+def test_batch_gradient_is_bit_reproducible_across_launch_shapes():
+    """Per-circuit gradient terms summed in circuit order: two runs, and runs
+    with a different number of resident workgroups (same block size), give
+    bit-identical gradients."""
+    say("qnn determinism")
+    from quict_amd.gradient.qnn import QNNBatch, synthetic_batch
+    widths, x, t = synthetic_batch(256, 16, 8, 14, 9)
+    theta = np.random.RandomState(3).uniform(-np.pi, np.pi, 4 * 2 * 14)
+    outs = []
+    for per_cu, graph in [(2, 1), (2, 1), (1, 0), (4, 1)]:
+        qb = QNNBatch(widths, x, t, layers=4, max_width=14,
+                      config={"lds_max_width": 11, "block_threads": 256, "blocks_per_cu": per_cu, "use_graph": graph})
+        loss, ev, g, gx = qb.forward_backward(theta)
+        outs.append((loss, ev.copy(), g.copy()))
+    for loss, ev, g in outs[1:]:
+        assert loss == outs[0][0]
+        assert np.array_equal(ev, outs[0][1])
+        assert np.array_equal(g, outs[0][2])
