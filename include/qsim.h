@@ -163,6 +163,20 @@ int qs_sv_reset(qs_ctx* ctx, void* vec, int n, int dtype, int bit, double p0, vo
 int qs_sv_sample(qs_ctx* ctx, const void* vec, int n, int dtype, const double* uniforms, int shots,
                  uint64_t* out, void* stream);
 
+/* No counterpart in the reference: qs_sv_sample for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  Same arguments, rule and outputs as
+   qs_sv_sample; its pyramid, uniforms and outcomes live in a fixed 40 MiB
+   device array per GPU (resolved once per device when the context is
+   created, held until the process ends) instead of the context's grow-on-
+   demand scratch, and it calls no hipMalloc / hipFree.  Limit: (2^n - 1) * 8
+   + 2048 + shots * (n + 1) * 8 bytes (rounded up to 256) must fit in 40 MiB,
+   i.e. at most 22 qubits; larger requests return QS_ERR_ARG.  Every call on
+   one GPU shares that GPU's array: calls that overlap on one GPU (several host
+   threads, or streams of other contexts of the same device) race. */
+int qs_sv_sample_noalloc(qs_ctx* ctx, const void* vec, int n, int dtype, const double* uniforms, int shots,
+                         uint64_t* out, void* stream);
+
 /* Same sampling rule over a caller device buffer of 2^n probabilities (used
    for the diagonal of a density matrix, DensityMatrixSimulator.sample,
    QuICT/simulation/density_matrix/density_matrix_simulator.py:210-240). */

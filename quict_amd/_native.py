@@ -82,6 +82,7 @@ SIGNATURES = {
     "qs_sv_collapse": (_I, [_P, _P, _I, _I, _I, _I, _D, _P]),
     "qs_sv_reset": (_I, [_P, _P, _I, _I, _I, _D, _P]),
     "qs_sv_sample": (_I, [_P, _P, _I, _I, _P, _I, _P, _P]),
+    "qs_sv_sample_noalloc": (_I, [_P, _P, _I, _I, _P, _I, _P, _P]),
     "qs_sample_probs": (_I, [_P, _P, _I, _P, _I, _P, _P]),
     "qs_sv_probabilities": (_I, [_P, _P, _I, _I, _P, _P]),
     "qs_sv_norm2": (_I, [_P, _P, _I, _I, ctypes.POINTER(_D), _P]),

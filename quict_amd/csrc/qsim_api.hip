@@ -307,6 +307,11 @@ qs_ctx* qs_ctx_create(int device) {
     delete ctx;
     return nullptr;
   }
+  if (noalloc_scratch_init(ctx) != QS_OK) {  // keeps its own error message
+    (void)hipHostFree(ctx->host_pinned);
+    delete ctx;
+    return nullptr;
+  }
   return ctx;
 }
 

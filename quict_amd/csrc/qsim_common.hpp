@@ -108,6 +108,9 @@ struct qs_ctx {
   uint64_t scratch_bytes = 0;
   // pinned host staging for small results
   double* host_pinned = nullptr;
+  // this device's copy of the fixed scratch of qs_sv_sample_noalloc (never freed)
+  void* noalloc_scratch = nullptr;
+  uint64_t noalloc_scratch_bytes = 0;
   // device program buffer for fused passes
   void* prog = nullptr;
   uint64_t prog_bytes = 0;

@@ -33,6 +33,10 @@ int reduce_prob_zero(qs_ctx* ctx, const void* vec, int n, int dtype, int bit, do
 bool dense_mfma_eligible(const qs_gate* g, int n);
 int launch_dense_mfma(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g, hipStream_t s);
 
+// Resolve this device's copy of the fixed sampling scratch (qsim_measure.hip);
+// called once per context, with the context's device current.
+int noalloc_scratch_init(qs_ctx* ctx);
+
 inline size_t amp_bytes(int dtype) { return dtype == QS_C128 ? 16 : 8; }
 
 }  // namespace qs

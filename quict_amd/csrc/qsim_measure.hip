@@ -200,23 +200,35 @@ template <typename R> static int reset_t(cx<R>* v, int n, int bit, double p0, hi
   return QS_OK;
 }
 
-template <typename Leaf>
-static int sample_t(qs_ctx* ctx, Leaf leaf, int n, const double* uniforms, int shots, uint64_t* out,
-                    hipStream_t s) {
-  // levels 1..n: sizes 2^(n-1) .. 1, stored back to back
-  std::vector<uint64_t> off(n + 1, 0);
+// Scratch layout of one sampling call: pyramid levels 1..n (sizes 2^(n-1) .. 1,
+// back to back), the level offsets, the uniforms and the outcomes.
+struct SampleLayout {
+  uint64_t off[64];
+  uint64_t pyr_bytes, off_bytes, u_bytes, o_bytes;
+  uint64_t total() const { return pyr_bytes + off_bytes + u_bytes + o_bytes; }
+};
+
+static SampleLayout sample_layout(int n, int shots) {
+  SampleLayout L{};
   uint64_t total = 0;
   for (int l = 1; l <= n; ++l) {
-    off[l] = total;
+    L.off[l] = total;
     total += 1ull << (n - l);
   }
-  const uint64_t pyr_bytes = ((total * sizeof(double)) + 255) & ~255ull;
-  const uint64_t off_bytes = 256 * 8;
-  const uint64_t u_bytes = (((uint64_t)shots * n * sizeof(double)) + 255) & ~255ull;
-  const uint64_t o_bytes = (uint64_t)shots * sizeof(uint64_t);
-  int rc = ensure_scratch(ctx, pyr_bytes + off_bytes + u_bytes + o_bytes);
-  if (rc) return rc;
-  char* base = (char*)ctx->scratch;
+  L.pyr_bytes = ((total * sizeof(double)) + 255) & ~255ull;
+  L.off_bytes = 256 * 8;
+  L.u_bytes = (((uint64_t)shots * n * sizeof(double)) + 255) & ~255ull;
+  L.o_bytes = (uint64_t)shots * sizeof(uint64_t);
+  return L;
+}
+
+// Build the pyramid and descend it for every shot, in `base` (at least
+// L.total() bytes of device memory).  Synchronous: returns with `out` filled.
+template <typename Leaf>
+static int sample_in(Leaf leaf, int n, const double* uniforms, int shots, uint64_t* out, const SampleLayout& L,
+                     char* base, hipStream_t s) {
+  const uint64_t* off = L.off;
+  const uint64_t pyr_bytes = L.pyr_bytes, off_bytes = L.off_bytes, u_bytes = L.u_bytes, o_bytes = L.o_bytes;
   double* pyr = (double*)base;
   uint64_t* d_off = (uint64_t*)(base + pyr_bytes);
   double* d_u = (double*)(base + pyr_bytes + off_bytes);
@@ -232,13 +244,42 @@ static int sample_t(qs_ctx* ctx, Leaf leaf, int n, const double* uniforms, int s
       QS_CHECK_HIP(hipGetLastError());
     }
   }
-  QS_CHECK_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_off, off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   QS_CHECK_HIP(hipMemcpyAsync(d_u, uniforms, (uint64_t)shots * n * sizeof(double), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_descend<Leaf>, dim3((unsigned)((shots + 255) / 256)), dim3(256), 0, s, leaf, pyr, d_off, n,
                      d_u, shots, d_out);
   QS_CHECK_HIP(hipGetLastError());
   QS_CHECK_HIP(hipMemcpyAsync(out, d_out, o_bytes, hipMemcpyDeviceToHost, s));
   QS_CHECK_HIP(hipStreamSynchronize(s));
+  return QS_OK;
+}
+
+template <typename Leaf>
+static int sample_t(qs_ctx* ctx, Leaf leaf, int n, const double* uniforms, int shots, uint64_t* out,
+                    hipStream_t s) {
+  const SampleLayout L = sample_layout(n, shots);
+  int rc = ensure_scratch(ctx, L.total());
+  if (rc) return rc;
+  return sample_in(leaf, n, uniforms, shots, out, L, (char*)ctx->scratch, s);
+}
+
+// Fixed device scratch of qs_sv_sample_noalloc: a file-scope array, one copy
+// per device, held until the process ends.  Host code never uses the name
+// itself as a device address: noalloc_scratch_init resolves each device's
+// copy with hipGetSymbolAddress (the context keeps it) and only that address
+// is passed to kernels and copies.
+constexpr uint64_t NOALLOC_SAMPLE_BYTES = 40ull << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SAMPLE_BYTES];
+
+int noalloc_scratch_init(qs_ctx* ctx) {
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return QS_ERR_DEVICE;
+  }
+  ctx->noalloc_scratch = ptr;
+  ctx->noalloc_scratch_bytes = NOALLOC_SAMPLE_BYTES;
   return QS_OK;
 }
 
@@ -300,6 +341,27 @@ int qs_sv_sample(qs_ctx* ctx, const void* vec, int n, int dtype, const double* u
   if (dtype == QS_C128)
     return sample_t(ctx, LeafState<double>{(const cx<double>*)vec}, n, uniforms, shots, out, as_stream(stream));
   return sample_t(ctx, LeafState<float>{(const cx<float>*)vec}, n, uniforms, shots, out, as_stream(stream));
+}
+
+int qs_sv_sample_noalloc(qs_ctx* ctx, const void* vec, int n, int dtype, const double* uniforms, int shots,
+                         uint64_t* out, void* stream) {
+  int rc = check_state_args(vec, n, dtype);
+  if (rc) return rc;
+  QS_REQUIRE(ctx && uniforms && out, "null argument");
+  QS_REQUIRE(shots >= 0, "negative shot count");
+  if (shots == 0) return QS_OK;
+  QS_REQUIRE(ctx->noalloc_scratch != nullptr, "qs_sv_sample_noalloc: the context has no fixed scratch");
+  const SampleLayout L = sample_layout(n, shots);
+  if (L.total() > ctx->noalloc_scratch_bytes) {
+    set_error("qs_sv_sample_noalloc: the pyramid of 2^n - 1 doubles plus shots * (n + 1) * 8 bytes exceeds the "
+              "fixed 40 MiB scratch (at most 22 qubits; fewer shots at 22)");
+    return QS_ERR_ARG;
+  }
+  char* base = (char*)ctx->noalloc_scratch;
+  if (dtype == QS_C128)
+    return sample_in(LeafState<double>{(const cx<double>*)vec}, n, uniforms, shots, out, L, base,
+                     as_stream(stream));
+  return sample_in(LeafState<float>{(const cx<float>*)vec}, n, uniforms, shots, out, L, base, as_stream(stream));
 }
 
 int qs_sample_probs(qs_ctx* ctx, const double* probs_dev, int n, const double* uniforms, int shots, uint64_t* out,
