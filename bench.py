@@ -123,7 +123,7 @@ def run_adjoint_20q(reps=3, cpu=False):
         # circuit through the C restatement (all cores) plus <psi|H|psi>
         from quict_amd.simulation.gate_compiler import compile_gates
         from oracle import cpu_baseline, gradients as og
-        threads = os.cpu_count() or 1
+        threads = min(os.cpu_count() or 1, 16)  # the host-CPU share of one GPU
         recs, keep = compile_gates(pc.bind(th).gates, n)
         v = np.zeros(1 << n, dtype=np.complex128)
         v[0] = 1
@@ -232,7 +232,7 @@ def cpu_dm14(c, nm, sample_records=6, dense_sizes=(5, 6, 7, 8)):
     recs, keep = program_records(nm.transpile(c, True).gates, n)
     rho = np.zeros(1 << (2 * n), dtype=np.complex128)
     rho[0] = 1
-    threads = os.cpu_count() or 1
+    threads = min(os.cpu_count() or 1, 16)  # the host-CPU share of one GPU
     k = min(sample_records, len(recs))
     t0 = time.perf_counter()
     cpu_baseline.apply_gates(rho, 2 * n, recs[:k], threads)
@@ -380,6 +380,38 @@ def nat_stream():
     return nat.stream_ptr(0)
 
 
+def touched_bytes(circuit, n, amp_bytes):
+    """Bytes the per-gate kernels read and write for the circuit: 2 * S per
+    amplitude a gate changes (k_gate inserts control bits as fixed ones and
+    phase / swap kernels visit only the amplitudes they change)."""
+    from quict_amd import _native as nat
+    from quict_amd.simulation.gate_compiler import compile_gates
+    recs, keep = compile_gates(circuit.gates, n)
+    total = 0
+    for r in recs:
+        c, t, kind = int(r["ncontrols"]), int(r["ntargets"]), int(r["kind"])
+        frac = 2.0 ** -c
+        if kind == nat.GATE_PHASE:
+            frac = 2.0 ** -(c + t)
+        elif kind == nat.GATE_SWAP:
+            frac /= 2
+        total += 2 * amp_bytes * frac * (1 << n)
+    return total
+
+
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
 def measured_traffic(n, ngates):
     """HBM bytes per tile-pass launch from the committed PMC summary (FETCH_SIZE +
     WRITE_SIZE, copy-calibrated) when this run is the workload it was measured on."""
@@ -407,7 +439,7 @@ def cpu_sv20(c20, shot_sample=4):
     n = 20
     recs, keep = compile_gates(c20.gates, n)
     out = {"kind": "port"}
-    for threads in (1, os.cpu_count() or 1):
+    for threads in (1, min(os.cpu_count() or 1, 16)):
         v = np.zeros(1 << n, dtype=np.complex128)
         v[0] = 1
         t0 = time.perf_counter()
@@ -503,11 +535,14 @@ def main():
             "tile_pass_gbs": (2 * (1 << n) * 8) / (ms64[0] / max(cnt64[0], 1) * 1e-3) / 1e9 if cnt64[0] else None}
         log("secondary: complex128 unfused (one pass per gate)")
         dtu, pu, msu, cntu = run_sv(n, circuit, "double", False, 1, 0, lib, ctx)
+        touched = touched_bytes(circuit, n, 16)
         secondary["c128_unfused"] = {
-            "ms_per_step": dtu * 1e3, "passes": pu, "gbs": pu * pass_bytes / dtu / 1e9,
-            "gates_per_s": args.gates / dtu,
-            "gate_pass_gbs": pass_bytes / (msu[1] / max(cntu[1], 1) * 1e-3) / 1e9 if cntu[1] else None,
-            "note": "gate passes touch only the amplitudes a gate changes (controls / phases touch 1/2 or 1/4)"}
+            "ms_per_step": dtu * 1e3, "passes": pu, "gates_per_s": args.gates / dtu,
+            "touched_gbs": touched / dtu / 1e9,
+            "touched_gbs_kernels": touched / (msu[1] * 1e-3) / 1e9 if cntu[1] else None,
+            "touched_bytes_per_step": touched,
+            "note": "one pass per gate; bytes are those each gate reads and writes (a control halves them, a "
+                    "phase gate touches 1/2^(controls+targets) of the state, a swap 1/2)"}
         log("secondary: 20-qubit 400-gate circuit + 1024 shots")
         c20 = workloads.sv20_circuit()
         dt20, p20, _, _ = run_sv(20, c20, "double", True, 5, 2, lib, ctx)
@@ -555,10 +590,12 @@ def main():
         gbs1, gps1, t1 = cpu_baseline_sample(28, 10, 1)
         cores = os.cpu_count() or 1
         gbsN, gpsN, tN = cpu_baseline_sample(28, 10, min(cores, 16))
-        cpu = {"value": gbs1, "unit": "GB/s", "cores": 1, "kind": "port",
+        cpu = {"value": gbs1, "unit": "GB/s", "cores": 1, "kind": "port", "host": host_cpu(),
                "sample": "first 10 gates of the same 60/40 seeded generator at 28 qubits (4 GiB complex128), "
                          "one pass per gate as the reference's numba kernels; " + f"{t1:.1f} s",
-               "gates_per_s": gps1, "all_cores": {"threads": min(cores, 16), "gbs": gbsN, "gates_per_s": gpsN}}
+               "gates_per_s": gps1,
+               "all_cores": {"threads": min(cores, 16), "gbs": gbsN, "gates_per_s": gpsN,
+                             "note": "threads capped at 16, the host-CPU share of one GPU on this machine"}}
 
     out = {
         "metric": METRIC,
@@ -582,6 +619,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": roof_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (roof_achieved / HBM_PEAK_GBS) if roof_achieved else None,
                      "traffic": measured_traffic(n, args.gates),
+                     "traffic_source": ("profiles/k_fused_sv30_c128_traffic.txt: rocprofv3 --pmc FETCH_SIZE and "
+                                        "WRITE_SIZE runs of this workload, copy-calibrated; not measured in this run")
+                     if measured_traffic(n, args.gates) else None,
                      "kernel": "k_fused<double,4,8>", "launches_per_step": fused_launches,
                      "avg_launch_ms": avg_fused_ms,
                      "bytes_per_launch": pass_bytes,

@@ -20,7 +20,13 @@
  *  - "bits" are amplitude-index bit positions (0 = least significant).  QuICT
  *    qubit q of an n-qubit register is bit n-1-q.
  *  - stream is a hipStream_t (NULL = the null stream).  Calls that return host
- *    values synchronise that stream; all others are asynchronous.
+ *    values synchronise that stream.  Gate calls (qs_sv_apply_gate(s),
+ *    qs_graph_launch) return once their kernels are queued, except a dense
+ *    gate on 3 or more targets run as its own pass (its matrix upload waits
+ *    for the stream); the fused program of a qs_sv_apply_gates call stays in
+ *    one of two per-context slots until its passes finish, so the call after
+ *    next on the same context may wait for them.  The adjoint, QNN and
+ *    circuit-matrix calls return host values and synchronise.
  *  - A context is bound to one device and is not thread-safe.
  */
 #ifndef QSIM_H

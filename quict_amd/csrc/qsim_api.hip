@@ -31,14 +31,30 @@ int ensure_scratch(qs_ctx* ctx, uint64_t bytes) {
   return QS_OK;
 }
 
-int ensure_prog(qs_ctx* ctx, uint64_t bytes) {
-  if (bytes <= ctx->prog_bytes) return QS_OK;
-  if (ctx->prog) QS_CHECK_HIP(hipFree(ctx->prog));
-  ctx->prog = nullptr;
-  ctx->prog_bytes = 0;
-  uint64_t want = std::max<uint64_t>(bytes, 1ull << 16);
-  QS_CHECK_HIP(hipMalloc(&ctx->prog, want));
-  ctx->prog_bytes = want;
+int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out) {
+  qs_ctx::ProgSlot& S = ctx->prog_slot[ctx->prog_next];
+  ctx->prog_next ^= 1;
+  if (!S.done) QS_CHECK_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
+  if (S.pending) {  // the call before last: its upload and passes must be finished
+    QS_CHECK_HIP(hipEventSynchronize(S.done));
+    S.pending = false;
+  }
+  const uint64_t want = std::max<uint64_t>(bytes, 1ull << 16);
+  if (S.host_bytes < bytes) {
+    if (S.host) QS_CHECK_HIP(hipHostFree(S.host));
+    S.host = nullptr;
+    S.host_bytes = 0;
+    QS_CHECK_HIP(hipHostMalloc((void**)&S.host, want, hipHostMallocDefault));
+    S.host_bytes = want;
+  }
+  if (S.dev_bytes < bytes) {
+    if (S.dev) QS_CHECK_HIP(hipFree(S.dev));
+    S.dev = nullptr;
+    S.dev_bytes = 0;
+    QS_CHECK_HIP(hipMalloc(&S.dev, want));
+    S.dev_bytes = want;
+  }
+  *out = &S;
   return QS_OK;
 }
 
@@ -320,7 +336,11 @@ void qs_ctx_destroy(qs_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
   if (ctx->scratch) (void)hipFree(ctx->scratch);
-  if (ctx->prog) (void)hipFree(ctx->prog);
+  for (auto& S : ctx->prog_slot) {
+    if (S.dev) (void)hipFree(S.dev);
+    if (S.host) (void)hipHostFree(S.host);
+    if (S.done) (void)hipEventDestroy(S.done);
+  }
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (auto& e : ctx->ev_pool) {
     (void)hipEventDestroy(e.first);
@@ -350,7 +370,9 @@ int qs_profile_read(qs_ctx* ctx, double* ms, int64_t* launches, int reset) {
   return QS_OK;
 }
 
-uint64_t qs_ctx_scratch_bytes(const qs_ctx* ctx) { return ctx ? ctx->scratch_bytes + ctx->prog_bytes : 0; }
+uint64_t qs_ctx_scratch_bytes(const qs_ctx* ctx) {
+  return ctx ? ctx->scratch_bytes + ctx->prog_slot[0].dev_bytes + ctx->prog_slot[1].dev_bytes : 0;
+}
 
 int qs_set_fusion(qs_ctx* ctx, int tile_log2, int low_bits) {
   QS_REQUIRE(ctx, "null ctx");

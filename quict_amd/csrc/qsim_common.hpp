@@ -111,9 +111,20 @@ struct qs_ctx {
   // this device's copy of the fixed scratch of qs_sv_sample_noalloc (never freed)
   void* noalloc_scratch = nullptr;
   uint64_t noalloc_scratch_bytes = 0;
-  // device program buffer for fused passes
-  void* prog = nullptr;
-  uint64_t prog_bytes = 0;
+  // fused-pass programs: two slots used in turn, each a pinned host staging
+  // buffer, a device buffer and an event recorded after the call's last pass,
+  // so a call returns without waiting and the next-but-one call reuses a slot
+  // only after its passes have finished
+  struct ProgSlot {
+    char* host = nullptr;
+    uint64_t host_bytes = 0;
+    void* dev = nullptr;
+    uint64_t dev_bytes = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  };
+  ProgSlot prog_slot[2];
+  int prog_next = 0;
   // kernel timing (HIP events recorded on the launch stream around each launch)
   int profile = 0;
   double prof_ms[4] = {0, 0, 0, 0};       // per kernel class, see qs_profile_read
@@ -125,7 +136,9 @@ struct qs_ctx {
 namespace qs {
 // Grow-only scratch buffers owned by the context.
 int ensure_scratch(qs_ctx* ctx, uint64_t bytes);
-int ensure_prog(qs_ctx* ctx, uint64_t bytes);
+// A free program slot of at least `bytes` (host and device), waiting for its
+// previous use to finish.
+int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out);
 int check_state_args(const void* vec, int n, int dtype);
 // Profiling: bracket a launch of kernel class `cls` with events (no-op unless enabled).
 int prof_begin(qs_ctx* ctx, int cls, hipStream_t s);

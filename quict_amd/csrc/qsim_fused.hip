@@ -700,16 +700,20 @@ static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int n
     if (dbg[0] == '1') print_program_stats(prog, n, RB + TB, L, ngates);
   ProgImage im;
   pack_program<R>(prog, im);
-  rc = ensure_prog(ctx, im.host.size() + 256);
+  // upload from a pinned slot that stays untouched until this call's passes
+  // have run (its event), so the call returns without waiting for the device
+  qs_ctx::ProgSlot* slot = nullptr;
+  rc = acquire_prog_slot(ctx, im.host.size() + 256, &slot);
   if (rc) return rc;
-  char* dprog = (char*)ctx->prog;
-  if (!im.host.empty())
-    QS_CHECK_HIP(hipMemcpyAsync(dprog, im.host.data(), im.host.size(), hipMemcpyHostToDevice, s));
+  char* dprog = (char*)slot->dev;
+  if (!im.host.empty()) {
+    std::memcpy(slot->host, im.host.data(), im.host.size());
+    QS_CHECK_HIP(hipMemcpyAsync(dprog, slot->host, im.host.size(), hipMemcpyHostToDevice, s));
+  }
   rc = launch_passes<R, RB, TB>(ctx, v, n, gates, prog, dprog, im, s, true, passes);
-  if (rc) return rc;
-  // the host staging buffer must outlive the async upload
-  QS_CHECK_HIP(hipStreamSynchronize(s));
-  return QS_OK;
+  QS_CHECK_HIP(hipEventRecord(slot->done, s));
+  slot->pending = true;
+  return rc;
 }
 
 }  // namespace qs

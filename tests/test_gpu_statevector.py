@@ -324,3 +324,72 @@ def test_ghz_support():
         s.run(c)
         counts = s.sample(100)
         assert counts[0] + counts[-1] == 100
+
+
+def test_fused_calls_return_before_their_passes_finish():
+    """qs_sv_apply_gates queues its fused passes and returns: right after a
+    29-qubit 300-gate call the stream still has work; three calls in a row
+    (program slots reused while earlier passes may still run) give the result
+    of one call on the concatenated gates and the oracle's, to 1e-12."""
+    say("async fused calls")
+    import ctypes
+    import torch
+    from quict_amd import _native as nat
+    from quict_amd.simulation.gate_compiler import compile_gates
+    lib = nat.load_library()
+    ctx = nat.context(0)
+    n = 29
+    v = torch.zeros(1 << n, dtype=torch.complex64, device="cuda:0")
+    v[0] = 1
+    c = workloads.random_mix_circuit(n, 300, 81, workloads.RANDOM_GATE_PROBS_60_40)
+    recs, keep = compile_gates(c.gates, n)
+    torch.cuda.synchronize()
+    nat.check(lib.qs_sv_apply_gates(ctx, ctypes.c_void_p(v.data_ptr()), n, nat.QS_C64, nat.gate_array_ptr(recs),
+                                    len(recs), 2, None, nat.stream_ptr(0)))
+    pending = not torch.cuda.current_stream().query()
+    torch.cuda.synchronize()
+    assert pending, "the call waited for its passes"
+    del v
+    n = 20
+    parts = [workloads.random_mix_circuit(n, 300, 82 + k, workloads.RANDOM_GATE_PROBS_60_40) for k in range(3)]
+    whole = Circuit(n)
+    for p in parts:
+        for g in p.gates:
+            whole.append(g)
+    outs = []
+    for chunks in ([whole], parts):
+        v = torch.zeros(1 << n, dtype=torch.complex128, device="cuda:0")
+        v[0] = 1
+        torch.cuda.synchronize()
+        for p in chunks:
+            r, kp = compile_gates(p.gates, n)
+            nat.check(lib.qs_sv_apply_gates(ctx, ctypes.c_void_p(v.data_ptr()), n, nat.QS_C128,
+                                            nat.gate_array_ptr(r), len(r), 1, None, nat.stream_ptr(0)))
+            del r, kp  # the records are read during the call
+        outs.append(v.cpu().numpy())
+    assert np.max(np.abs(outs[0] - outs[1])) < 1e-12
+    assert np.max(np.abs(outs[1] - osv.run(whole))) < 1e-12
+
+
+def test_sample_above_24_qubits_returns_nonzero_counts():
+    """Above 24 qubits sample() returns {basis index: count} of the outcomes
+    that occurred (a 2^n list is out of reach at 30 qubits: 2^30 Python ints);
+    its entries equal the seeded per-shot outcomes, and the Simulator result
+    keys them by n-character bit strings as the reference's Result does."""
+    say("sample dict above 24 qubits")
+    from quict_amd.simulation import StateVectorSimulator, Simulator
+    n = 25
+    c = workloads.random_mix_circuit(n, 60, 91)
+    sim = StateVectorSimulator("GPU", precision="single")
+    sim.run(c)
+    random.seed(17)
+    counts = sim.sample(500)
+    assert isinstance(counts, dict) and sum(counts.values()) == 500
+    random.seed(17)
+    idx = sim.sample_indices(500)
+    vals, cnt = np.unique(idx, return_counts=True)
+    assert counts == {int(v): int(k) for v, k in zip(vals, cnt)}
+    random.seed(17)
+    res = Simulator(device="GPU", backend="state_vector", precision="single").run(c, shots=500)
+    assert res["shots"] == 500
+    assert res["data"]["counts"] == {format(k, "b").zfill(n): v for k, v in sorted(counts.items())}
