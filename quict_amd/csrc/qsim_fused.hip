@@ -265,42 +265,20 @@ __device__ __forceinline__ void do_densek(cx<R> (&a)[NR], const cx<R>* m, uint32
   }
 }
 
-// Diagonal gate that depends on thread bits only: its value is a per-lane
-// scalar, which commutes with every in-register op (those mix registers of one
-// lane), so it is folded into the lane factor `f` and applied to the registers
-// only before a relayout or the store.
-// Selector of a diagonal table: up to two thread-index bits t0 (MSB), t1.
+// Diagonal gate that depends on thread bits only: one complex factor per lane
+// (table entry `sel`, up to two thread-index bits t0 (MSB), t1), applied to every
+// register at once.  One per-lane LDS read instead of selects between table
+// entries: selects of loaded values become lane-divergent branches.
 __device__ __forceinline__ uint32_t diag_sel(int ns, int t0, int t1, uint32_t tid) {
   const uint32_t b0 = (tid >> t0) & 1u, b1 = (tid >> t1) & 1u;
   return ns == 0 ? 0u : ns == 1 ? b0 : ((b0 << 1) | b1);
 }
 
-template <typename R>
-__device__ __forceinline__ void diag_lane(cx<R>& f, int ns, uint32_t sel, const cx<R>* T) {
-  // component-wise selects: selecting whole structs by a lane value makes the
-  // compiler index a private (scratch) array
-  R re = T[0].re, im = T[0].im;
-  if (ns >= 1) {
-    const bool b0 = sel & 1u;
-    if (ns == 1) {
-      re = b0 ? T[1].re : re;
-      im = b0 ? T[1].im : im;
-    } else {
-      const bool b1 = sel & 2u;
-      const R lo_re = b0 ? T[1].re : re, lo_im = b0 ? T[1].im : im;
-      const R hi_re = b0 ? T[3].re : T[2].re, hi_im = b0 ? T[3].im : T[2].im;
-      re = b1 ? hi_re : lo_re;
-      im = b1 ? hi_im : lo_im;
-    }
-  }
-  f = mul(mk<R>(re, im), f);
-}
-
 template <typename R, int NR>
-__device__ __forceinline__ void flush_lane(cx<R> (&a)[NR], cx<R>& f) {
+__device__ __forceinline__ void do_diag_lane(cx<R> (&a)[NR], uint32_t sel, const cx<R>* T) {
+  const cx<R> f = ld(T + sel);
 #pragma unroll
   for (int r = 0; r < NR; ++r) a[r] = mul(f, a[r]);
-  f = mk<R>((R)1, (R)0);
 }
 
 // Diagonal op: the host resolved every register-bit dependence at plan time, so
@@ -347,17 +325,13 @@ __device__ __forceinline__ void relayout_half(cx<R> (&a)[1 << RB], char* lds, co
 
 // One op of a pass program applied to the register tile.
 template <typename R, int RB, int TB, bool DK>
-__device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdirty, char* lds, const FOpW& e,
+__device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], char* lds, const FOpW& e,
                                        const cx<R>* __restrict__ C, const cx<R>* __restrict__ G,
                                        const uint32_t* __restrict__ X, uint32_t tid) {
   constexpr int NR = 1 << RB;
   const int kind = (int)(e.w[0] & 0xff);
   const uint32_t opcoef = e.w[3];
   if (kind == F_RELAYOUT) {
-    if (fdirty) {
-      flush_lane<R, NR>(a, f);
-      fdirty = false;
-    }
     const uint32_t* x = X + opcoef;
     const uint32_t pk = x[0];
     if (pk == 0) relayout_half<R, RB, TB, 0>(a, lds, x, tid);
@@ -388,37 +362,41 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
     if constexpr (P < RB) {                                           \
       if (M == 0 && pure) do_dense1_real<R, NR, P>(a, g);              \
       else do_dense1<R, NR, P, M>(a, g, creg, cregv, tok);             \
+    } else {                                                          \
+      __builtin_unreachable();                                        \
     }                                                                 \
     break;
 #define QS_D1P(P) QS_D1(P, 0) QS_D1(P, 1) QS_D1(P, 2) QS_D1(P, 3)
         QS_D1P(0) QS_D1P(1) QS_D1P(2) QS_D1P(3) QS_D1P(4)
 #undef QS_D1P
 #undef QS_D1
+        default:
+          __builtin_unreachable();
       }
       break;
     case F_PERM1:
       switch (pos0 * 4 + cm) {
-#define QS_P1(P, M) \
-  case P * 4 + M: \
-    if constexpr (P < RB) {                                                                   \
-      if (pure) do_perm1<R, NR, P, M, true>(a, g, creg, cregv, tok);                           \
-      else do_perm1<R, NR, P, M, false>(a, g, creg, cregv, tok);                               \
-    }                                                                                         \
+#define QS_P1(P, M)                                                   \
+  case P * 4 + M:                                                     \
+    if constexpr (P < RB) {                                           \
+      if (pure) do_perm1<R, NR, P, M, true>(a, g, creg, cregv, tok);   \
+      else do_perm1<R, NR, P, M, false>(a, g, creg, cregv, tok);       \
+    } else {                                                          \
+      __builtin_unreachable();                                        \
+    }                                                                 \
     break;
 #define QS_P1P(P) QS_P1(P, 0) QS_P1(P, 1) QS_P1(P, 2) QS_P1(P, 3)
         QS_P1P(0) QS_P1P(1) QS_P1P(2) QS_P1P(3) QS_P1P(4)
 #undef QS_P1P
 #undef QS_P1
+        default:
+          __builtin_unreachable();
       }
       break;
     case F_DIAG: {
       const uint32_t sel = diag_sel(ntt, (int)(e.w[2] & 0xff), (int)((e.w[2] >> 8) & 0xff), tid);
-      if (pure) {
-        diag_lane<R>(f, ntt, sel, m);
-        fdirty = true;
-      } else {
-        do_diag<R, NR>(a, sel, m);
-      }
+      if (pure) do_diag_lane<R, NR>(a, sel, m);
+      else do_diag<R, NR>(a, sel, m);
       break;
     }
     case F_DENSE2:
@@ -448,11 +426,15 @@ __device__ __forceinline__ void run_op(cx<R> (&a)[1 << RB], cx<R>& f, bool& fdir
         if (ctl) do_dense2<R, NR, A, B, 3>(a, g, creg, cregv, tok);                  \
         else do_dense2<R, NR, A, B, 0>(a, g, creg, cregv, tok);                      \
       }                                                                              \
+    } else {                                                                         \
+      __builtin_unreachable();                                                       \
     }                                                                                \
     break;
         QS_D2(1, 0) QS_D2(2, 0) QS_D2(2, 1) QS_D2(3, 0) QS_D2(3, 1) QS_D2(3, 2)
         QS_D2(4, 0) QS_D2(4, 1) QS_D2(4, 2) QS_D2(4, 3)
 #undef QS_D2
+        default:
+          __builtin_unreachable();
       }
       break;
     }
@@ -486,23 +468,35 @@ case M:        \
       break;
     }
     default:
-      break;
+      __builtin_unreachable();
   }
 }
 
 // Apply a pass's op list to the register tile `a` (load layout on entry, store
 // layout on exit).  DK: include the in-register 3- and 4-target dense ops.
-template <typename R, int RB, int TB, bool DK>
+// U2: two ops per loop iteration.  With one op per iteration the tile's value at
+// the loop head and after the op switch get different registers (an op writes
+// its outputs while its inputs are still live), and the compiler copies the
+// whole tile (64 registers at complex128) twice per op; with two ops per
+// iteration the first op's outputs form their own register set and neither op
+// needs a copy.  Every impossible switch arm is unreachable (the host checks
+// each op before launch), so no arm passes the tile through unchanged.
+template <typename R, int RB, int TB, bool DK, bool U2>
 __device__ __forceinline__ void run_ops(cx<R> (&a)[1 << RB], char* lds, const PassHdr* __restrict__ H,
                                         const FOpW* __restrict__ P, const cx<R>* __restrict__ C,
                                         const cx<R>* __restrict__ G, const uint32_t* __restrict__ X,
                                         uint32_t tid) {
-  constexpr int NR = 1 << RB;
   const int nops = H->nops;
-  cx<R> f = mk<R>((R)1, (R)0);  // deferred per-lane diagonal factor
-  bool fdirty = false;
-  for (int o = 0; o < nops; ++o) run_op<R, RB, TB, DK>(a, f, fdirty, lds, P[o], C, G, X, tid);
-  if (fdirty) flush_lane<R, NR>(a, f);
+  if constexpr (U2) {
+    int o = 0;
+    for (; o + 1 < nops; o += 2) {
+      run_op<R, RB, TB, DK>(a, lds, P[o], C, G, X, tid);
+      run_op<R, RB, TB, DK>(a, lds, P[o + 1], C, G, X, tid);
+    }
+    if (o < nops) run_op<R, RB, TB, DK>(a, lds, P[o], C, G, X, tid);
+  } else {
+    for (int o = 0; o < nops; ++o) run_op<R, RB, TB, DK>(a, lds, P[o], C, G, X, tid);
+  }
 }
 
 // One pass over all 2^(n-K) tiles.  PF: persistent workgroups that prefetch
@@ -586,7 +580,7 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         a[r] = ld_nt(v + (base | g));
       }
     }
-    run_ops<R, RB, TB, DK>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
+    run_ops<R, RB, TB, DK, !DK && !PF>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
     if (H->sync_store) __syncthreads();
     // store: tile base (uniform) XOR per-lane part (with pending flips) XOR register part
     const uint64_t base = insert_zeros(t, H->sorted, K) ^ gout;

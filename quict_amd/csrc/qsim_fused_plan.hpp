@@ -1151,7 +1151,12 @@ static int schedule_pass(const qs_gate* gates, PassPlan& p, const Config& c, uin
   };
   auto emit = [&](size_t k) -> int {
     const qs_gate& g = gates[p.gates[k]];
-    if (relabel_swap(g, loc[k])) {
+    // a plain SWAP of two register slots also relabels: no bit moves between
+    // registers and threads, so the load/store layouts are unaffected
+    const bool reg_swap = g.kind == QS_GATE_SWAP && g.ncontrols == 0 && g.m[0] == 1.0 && g.m[1] == 0.0 &&
+                          g.m[2] == 1.0 && g.m[3] == 0.0 && index_in(cur.rs, loc[k][0]) >= 0 &&
+                          index_in(cur.rs, loc[k][1]) >= 0;
+    if (relabel_swap(g, loc[k]) || reg_swap) {
       // the two slots exchange the tile bits they carry; flips stay with the slots
       const int a = loc[k][0], b = loc[k][1];
       auto& va = (index_in(cur.rs, a) >= 0) ? cur.rs : cur.ts;
@@ -1344,8 +1349,45 @@ static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, b
   return QS_OK;
 }
 
-static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
-                         FusedProgram& prog) {
+// The tile kernel's op switch treats every encoding outside these ranges as
+// unreachable: a program is checked here before it can reach a launch.
+static int check_program(const FusedProgram& prog, const Config& c) {
+  for (const PassHdr& h : prog.hdrs) {
+    for (int i = 0; i < h.nops; ++i) {
+      const FOp& o = prog.ops[(size_t)h.op0 + i];
+      bool ok = o.kind >= F_RELAYOUT && o.kind <= F_XX2;
+      switch (o.kind) {
+        case F_DENSE1:
+        case F_PERM1:
+          ok = o.pos[0] >= 0 && o.pos[0] < c.RB;
+          break;
+        case F_DIAG:
+          ok = o.ntt >= 0 && o.ntt <= 2 && (o.ntt < 1 || (o.tpos[0] >= 0 && o.tpos[0] < c.TB)) &&
+               (o.ntt < 2 || (o.tpos[1] >= 0 && o.tpos[1] < c.TB));
+          break;
+        case F_DENSE2:
+        case F_SWAP2:
+        case F_XX2:
+          ok = o.pos[1] >= 0 && o.pos[0] > o.pos[1] && o.pos[0] < c.RB;
+          break;
+        case F_DENSE3:
+        case F_DENSE4:
+          ok = h.has_dk != 0;
+          break;
+        default:
+          break;
+      }
+      if (!ok) {
+        set_error("fusion planner: internal error: op outside the tile kernel's encodings");
+        return QS_ERR_ARG;
+      }
+    }
+  }
+  return QS_OK;
+}
+
+static int build_program_unchecked(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
+                                   FusedProgram& prog) {
   // every tile keeps its L low bits on thread slots at the load and the store
   if (c.L < 1 || c.L > c.TB) {
     set_error("fusion planner: low_bits must be in [1, tile_log2 - 4]");
@@ -1362,6 +1404,12 @@ static int build_program(const qs_gate* gates, int ngates, int n, const Config& 
   const size_t a = moved.plans.size(), b = prog.plans.size();
   if (c.remap == 2 ? a < b : 100 * a <= 92 * b) prog = std::move(moved);
   return QS_OK;
+}
+
+static int build_program(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
+                         FusedProgram& prog) {
+  const int rc = build_program_unchecked(gates, ngates, n, c, amp_bytes, prog);
+  return rc ? rc : check_program(prog, c);
 }
 
 static void print_program_stats(const FusedProgram& prog, int n, int K, int L, int ngates) {
