@@ -524,25 +524,63 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
   } else {
     C = coef + H->coef0;
   }
-  // per-lane parts of the load and store addresses, per-register parts as bit masks
-  uint64_t gin = 0, gout = 0;
-#pragma unroll
-  for (int b = 0; b < TB; ++b) {
-    const uint64_t bit = (tid >> b) & 1u;
-    gin |= bit << H->tbits[H->ts0[b]];
-    gout |= bit << H->tbo[H->ts1[b]];
-  }
-  gout ^= H->sflip;
-  uint64_t min_[RB], mout[RB];
-#pragma unroll
-  for (int b = 0; b < RB; ++b) {
-    min_[b] = 1ull << H->tbits[H->rs0[b]];
-    mout[b] = 1ull << H->tbo[H->rs1[b]];
-  }
   cx<R> a[NR];
-  [[maybe_unused]] cx<R> nx[PF ? NR : 1];
-  uint64_t t = blockIdx.x;
-  if constexpr (PF) {
+  if constexpr (!PF) {
+    // One tile per workgroup, straight-line: the store address is derived from
+    // the header after the ops, so only the tile base stays live across them
+    // (the op interpreter needs every scalar register it can get).
+    const uint64_t tb = insert_zeros((uint64_t)blockIdx.x, H->sorted, K);
+    {
+      uint64_t gin = 0;
+#pragma unroll
+      for (int b = 0; b < TB; ++b) gin |= (uint64_t)((tid >> b) & 1u) << H->tbits[H->ts0[b]];
+      uint64_t min_[RB];
+#pragma unroll
+      for (int b = 0; b < RB; ++b) min_[b] = 1ull << H->tbits[H->rs0[b]];
+      const uint64_t base = tb | gin;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        uint64_t g = 0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+          if ((r >> b) & 1) g |= min_[b];
+        a[r] = ld_nt(v + (base | g));
+      }
+    }
+    run_ops<R, RB, TB, DK, !DK>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
+    if (H->sync_store) __syncthreads();
+    // store: tile base (uniform) XOR per-lane part (with pending flips) XOR register part
+    uint64_t gout = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b) gout |= (uint64_t)((tid >> b) & 1u) << H->tbo[H->ts1[b]];
+    const uint64_t base = tb ^ gout ^ H->sflip;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      uint64_t g = 0;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+        if ((r >> b) & 1) g |= 1ull << H->tbo[H->rs1[b]];
+      st_nt(v + (base ^ g), a[r]);
+    }
+    (void)ntiles;
+  } else {
+    // per-lane parts of the load and store addresses, per-register parts as bit masks
+    uint64_t gin = 0, gout = 0;
+#pragma unroll
+    for (int b = 0; b < TB; ++b) {
+      const uint64_t bit = (tid >> b) & 1u;
+      gin |= bit << H->tbits[H->ts0[b]];
+      gout |= bit << H->tbo[H->ts1[b]];
+    }
+    gout ^= H->sflip;
+    uint64_t min_[RB], mout[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      min_[b] = 1ull << H->tbits[H->rs0[b]];
+      mout[b] = 1ull << H->tbo[H->rs1[b]];
+    }
+    cx<R> nx[NR];
+    uint64_t t = blockIdx.x;
     if (t < ntiles) {
       const uint64_t base = insert_zeros(t, H->sorted, K) | gin;
 #pragma unroll
@@ -554,9 +592,7 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
         a[r] = ld_nt(v + (base | g));
       }
     }
-  }
-  for (; t < ntiles; t += gridDim.x) {
-    if constexpr (PF) {
+    for (; t < ntiles; t += gridDim.x) {
       const uint64_t tn = t + gridDim.x;
       if (tn < ntiles) {
         const uint64_t base = insert_zeros(tn, H->sorted, K) | gin;
@@ -569,30 +605,17 @@ __global__ __launch_bounds__(1 << TB) void k_fused(cx<R>* __restrict__ v, const 
           nx[r] = ld_nt(v + (base | g));
         }
       }
-    } else {
-      const uint64_t base = insert_zeros(t, H->sorted, K) | gin;
+      run_ops<R, RB, TB, DK, false>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
+      if (H->sync_store) __syncthreads();
+      const uint64_t base = insert_zeros(t, H->sorted, K) ^ gout;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         uint64_t g = 0;
 #pragma unroll
         for (int b = 0; b < RB; ++b)
-          if ((r >> b) & 1) g |= min_[b];
-        a[r] = ld_nt(v + (base | g));
+          if ((r >> b) & 1) g |= mout[b];
+        st_nt(v + (base ^ g), a[r]);
       }
-    }
-    run_ops<R, RB, TB, DK, !DK && !PF>(a, lds_raw, H, P, C, coef + H->coef0, aux, tid);
-    if (H->sync_store) __syncthreads();
-    // store: tile base (uniform) XOR per-lane part (with pending flips) XOR register part
-    const uint64_t base = insert_zeros(t, H->sorted, K) ^ gout;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      uint64_t g = 0;
-#pragma unroll
-      for (int b = 0; b < RB; ++b)
-        if ((r >> b) & 1) g |= mout[b];
-      st_nt(v + (base ^ g), a[r]);
-    }
-    if constexpr (PF) {
 #pragma unroll
       for (int r = 0; r < NR; ++r) a[r] = nx[r];
     }
