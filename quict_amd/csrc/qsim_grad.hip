@@ -14,6 +14,7 @@
 #include "qsim_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -256,6 +257,162 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_multi(cx<R>* __restrict__ p
   }
 }
 
+// ---- many consecutive steps on a tile of both vectors in LDS -----------------
+// A pass: up to AT_MAX_STEPS backward steps (AT_MAX_GRADS of them
+// parameterised) whose bits, together with the AT_LOW lowest amplitude bits,
+// fit in a tile of nb <= AT_MAX_BITS amplitude bits.  A workgroup loads the
+// tile of psi and lam into LDS (runs of 2^AT_LOW contiguous amplitudes),
+// applies every step there (one workgroup barrier per step), reduces each
+// parameterised step's Re <lam|dG|psi> per wave into LDS and stores the tile
+// once.  (Runs of steps on <= 3 bits applied in registers per LDS round trip
+// were slower: half the threads idle on 1024-amplitude tiles.)
+constexpr int AT_MAX_BITS = 11;
+constexpr int AT_MAX_STEPS = 128;
+constexpr int AT_MAX_GRADS = 64;
+constexpr int AT_LOW = 3;
+
+struct TStep {
+  int32_t nt;            // 1 or 2 targets
+  int32_t p0, p1;        // tile-local positions of target 0 (matrix MSB) and target 1
+  int32_t gi;            // parameterised: accumulator slot in the pass, else -1
+  uint32_t cmask, cval;  // tile-local control bits and the values they need
+  int32_t pad[2];
+  double g[32];          // G^dagger, (re, im), 2^nt x 2^nt
+  double d[32];          // dG/dtheta
+};
+
+struct TPass {
+  int32_t ubits[AT_MAX_BITS];  // amplitude bits of the tile, ascending (tile-local bit b = ubits[b])
+  int32_t nb;
+  int32_t s0, ns;              // steps [s0, s0 + ns) of the step table, in application order
+  int32_t ngrad;
+  int32_t row[AT_MAX_GRADS];   // partial row of accumulator slot k
+};
+
+// LDS slot of tile amplitude i: the low 4 index bits XOR index bits 4-7 and
+// 8-11 (a bijection), so that lanes whose pairs are spaced 2, 4 or 8
+// amplitudes apart spread over the banks.
+__device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15u); }
+
+// Insert a zero bit at local position p.
+__device__ __forceinline__ uint32_t ins0(uint32_t q, int p) {
+  const uint32_t lo = q & ((1u << p) - 1u);
+  return ((q - lo) << 1) | lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ psi, cx<R>* __restrict__ lam,
+                                                          uint64_t ntiles, const TPass P,
+                                                          const TStep* __restrict__ steps,
+                                                          double* __restrict__ partial) {
+  extern __shared__ __align__(16) char at_lds[];
+  const int nb = P.nb;
+  const uint32_t NA = 1u << nb;
+  cx<R>* sp = reinterpret_cast<cx<R>*>(at_lds);
+  cx<R>* sl = sp + NA;
+  double* red = reinterpret_cast<double*>(sl + NA);  // [AT_MAX_GRADS][ADJ_THREADS / 64]
+  const uint32_t tid = threadIdx.x;
+  const int wave = (int)(tid >> 6);
+  constexpr int NW = ADJ_THREADS / 64;
+  for (int i = (int)tid; i < AT_MAX_GRADS * NW; i += ADJ_THREADS) red[i] = 0.0;
+  // address of local index i = tid + 256 k: deposit of the low 8 bits (per
+  // thread, once) OR deposit of the bits above
+  uint64_t dlo = 0;
+  for (int b = 0; b < nb && b < 8; ++b)
+    if ((tid >> b) & 1u) dlo |= 1ull << P.ubits[b];
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = insert_zeros(t, P.ubits, nb);
+    __syncthreads();  // the previous tile's stores have read the LDS image
+    for (uint32_t i = tid; i < NA; i += ADJ_THREADS) {
+      uint64_t a = base | dlo;
+      for (int b = 8; b < nb; ++b)
+        if ((i >> b) & 1u) a |= 1ull << P.ubits[b];
+      sp[swz(i)] = ld(psi + a);
+      sl[swz(i)] = ld(lam + a);
+    }
+    __syncthreads();
+    for (int k = 0; k < P.ns; ++k) {
+      const TStep& S = steps[P.s0 + k];
+      double acc = 0.0;
+      const bool grad = S.gi >= 0;
+      if (S.nt == 1) {
+        const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
+        for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
+          const uint32_t i0 = ins0(q, S.p0), i1 = i0 | (1u << S.p0);
+          if ((i0 & S.cmask) != S.cval) continue;
+          const uint32_t j0 = swz(i0), j1 = swz(i1);
+          const cx<R> a0 = sp[j0], a1 = sp[j1], b0 = sl[j0], b1 = sl[j1];
+          const cx<R> pa = cfma(mul(g00, a0), g01, a1), pb = cfma(mul(g10, a0), g11, a1);
+          sl[j0] = cfma(mul(g00, b0), g01, b1);
+          sl[j1] = cfma(mul(g10, b0), g11, b1);
+          sp[j0] = pa;
+          sp[j1] = pb;
+          if (grad) {
+            const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), pa), cget<R>(S.d, 1), pb);
+            const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), pa), cget<R>(S.d, 3), pb);
+            acc += (double)b0.re * (double)m0.re + (double)b0.im * (double)m0.im + (double)b1.re * (double)m1.re +
+                   (double)b1.im * (double)m1.im;
+          }
+        }
+      } else {
+        const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
+        for (uint32_t q = tid; q < NA / 4; q += ADJ_THREADS) {
+          const uint32_t i = ins0(ins0(q, plo), phi);
+          if ((i & S.cmask) != S.cval) continue;
+          const uint32_t idx[4] = {swz(i), swz(i | (1u << S.p1)), swz(i | (1u << S.p0)),
+                                   swz(i | (1u << S.p0) | (1u << S.p1))};
+          cx<R> a[4], b[4], pa[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            a[r] = sp[idx[r]];
+            b[r] = sl[idx[r]];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            cx<R> x = mul(cget<R>(S.g, 4 * r), a[0]), y = mul(cget<R>(S.g, 4 * r), b[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) {
+              x = cfma(x, cget<R>(S.g, 4 * r + c), a[c]);
+              y = cfma(y, cget<R>(S.g, 4 * r + c), b[c]);
+            }
+            pa[r] = x;
+            sp[idx[r]] = x;
+            sl[idx[r]] = y;
+          }
+          if (grad) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              cx<R> mu = mul(cget<R>(S.d, 4 * r), pa[0]);
+#pragma unroll
+              for (int c = 1; c < 4; ++c) mu = cfma(mu, cget<R>(S.d, 4 * r + c), pa[c]);
+              acc += (double)b[r].re * (double)mu.re + (double)b[r].im * (double)mu.im;
+            }
+          }
+        }
+      }
+      if (grad) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if ((tid & 63u) == 0) red[S.gi * NW + wave] += acc;  // one writer per (slot, wave)
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = tid; i < NA; i += ADJ_THREADS) {
+      uint64_t a = base | dlo;
+      for (int b = 8; b < nb; ++b)
+        if ((i >> b) & 1u) a |= 1ull << P.ubits[b];
+      st(psi + a, sp[swz(i)]);
+      st(lam + a, sl[swz(i)]);
+    }
+  }
+  __syncthreads();
+  if ((int)tid < P.ngrad) {
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w) s += red[tid * NW + w];
+    partial[(uint64_t)P.row[tid] * ADJ_MAX_BLOCKS + blockIdx.x] = s;
+  }
+}
+
 // sums[k] = sum_b partial[k * stride + b], b < nb[k]
 __global__ void k_adj_reduce(const double* __restrict__ partial, const int* __restrict__ nb, int stride, int nsteps,
                              double* __restrict__ sums) {
@@ -309,27 +466,85 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   int rc = QS_OK;
   std::vector<int> nb;
   std::vector<int> step_slot;
-  // Plan: runs of consecutive steps (backward order) spanning <= 3 amplitude
-  // bits become one multi-gate pass; other gates run as single steps.
+  // Plan: runs of consecutive steps (backward order) become one pass: tile
+  // passes in LDS (default: up to AT_MAX_STEPS steps whose bits fit in a tile of
+  // `tile_bits` amplitude bits with the AT_LOW lowest ones), or register passes
+  // spanning <= 3 amplitude bits (QS_ADJ_TILE=0); other gates run as single steps.
   struct Item {
-    int multi;   // 1: pass index into passes, 0: single gate index
+    int multi;   // 1: register pass index into passes, 2: tile pass index into tpasses, 0: single gate index
     int idx;
   };
   std::vector<Item> items;
   std::vector<MPass> passes;
   std::vector<MGate> mg;
+  std::vector<TPass> tpasses;
+  std::vector<TStep> ts;
+  const char* tenv = getenv("QS_ADJ_TILE");
+  const bool tile = !(tenv && tenv[0] == '0');
+  int tile_bits = std::min(n, sizeof(R) == 8 ? 10 : 11);
+  if (const char* tb = getenv("QS_ADJ_TILE_BITS")) tile_bits = std::min(n, std::max(AT_LOW + 2, std::min(AT_MAX_BITS, atoi(tb))));
+  const uint64_t lowmask = (1ull << std::min(n, AT_LOW)) - 1;
   std::vector<int> cur;  // gate indices of the open pass, in application order
   uint64_t curbits = 0;
-  auto eligible = [&](int j) {
-    const qs_gate& g = gates[j];
-    if (g.ntargets > 2 || g.ncontrols + g.ntargets > 3) return false;
-    if (slot[j] >= 0 && (dgates[j].ntargets != g.ntargets || dgates[j].ncontrols != g.ncontrols)) return false;
-    return true;
-  };
   auto gbits = [&](int j) {
     uint64_t m = 0;
     for (int i = 0; i < gates[j].ncontrols + gates[j].ntargets; ++i) m |= 1ull << gates[j].bits[i];
     return m;
+  };
+  auto eligible = [&](int j) {
+    const qs_gate& g = gates[j];
+    if (g.ntargets > 2) return false;
+    if (tile ? __builtin_popcountll(gbits(j) | lowmask) > tile_bits : g.ncontrols + g.ntargets > 3) return false;
+    if (slot[j] >= 0 && (dgates[j].ntargets != g.ntargets || dgates[j].ncontrols != g.ncontrols)) return false;
+    return true;
+  };
+  auto flush_tile = [&]() {
+    TPass P;
+    std::memset(&P, 0, sizeof(P));
+    uint64_t u = curbits | lowmask;
+    for (int b = 0; b < n && __builtin_popcountll(u) < tile_bits; ++b) u |= 1ull << b;  // fill: longer runs
+    int nbits = 0;
+    for (int b = 0; b < n; ++b)
+      if ((u >> b) & 1) P.ubits[nbits++] = b;
+    P.nb = nbits;
+    P.s0 = (int)ts.size();
+    P.ns = (int)cur.size();
+    const uint64_t ntiles = 1ull << (n - nbits);
+    const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
+    auto local = [&](int bit) {
+      for (int i = 0; i < nbits; ++i)
+        if (P.ubits[i] == bit) return i;
+      return -1;
+    };
+    for (int j : cur) {
+      const qs_gate& g = gates[j];
+      TStep S;
+      std::memset(&S, 0, sizeof(S));
+      S.nt = g.ntargets;
+      S.p0 = local(g.bits[g.ncontrols]);
+      S.p1 = g.ntargets > 1 ? local(g.bits[g.ncontrols + 1]) : 0;
+      S.gi = -1;
+      for (int i = 0; i < g.ncontrols; ++i) {
+        S.cmask |= 1u << local(g.bits[i]);
+        S.cval |= 1u << local(g.bits[i]);
+      }
+      std::vector<double> M;
+      record_dense(g, M);
+      for (size_t i = 0; i < M.size(); ++i) S.g[i] = M[i];
+      if (slot[j] >= 0) {
+        record_dense(dgates[j], M);
+        for (size_t i = 0; i < M.size(); ++i) S.d[i] = M[i];
+        S.gi = P.ngrad;
+        P.row[P.ngrad++] = (int)nb.size();
+        nb.push_back(blocks);
+        step_slot.push_back(slot[j]);
+      }
+      ts.push_back(S);
+    }
+    items.push_back({2, (int)tpasses.size()});
+    tpasses.push_back(P);
+    cur.clear();
+    curbits = 0;
   };
   auto flush = [&]() {
     if (cur.empty()) return;
@@ -378,7 +593,61 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     curbits = 0;
   };
   std::vector<int> single_row(ngates, -1);
+  // Tile passes group the steps like the forward fusion planner: within a run
+  // of eligible steps (backward order), a step may join the current pass ahead
+  // of steps still waiting if it shares no bit with them (disjoint gates
+  // commute, so the gradient of the same function comes out), fewest new tile
+  // bits first; the pass applies its steps in backward order.
+  std::vector<int> run;
+  auto plan_run = [&]() {
+    std::vector<char> done(run.size(), 0);
+    size_t left = run.size();
+    while (left) {
+      uint64_t S = lowmask;
+      int ng = 0, np = 0;
+      std::vector<char> pick(run.size(), 0);
+      while (np < AT_MAX_STEPS) {
+        uint64_t wait = 0;
+        int best = -1, best_new = 1 << 30;
+        for (size_t k = 0; k < run.size(); ++k) {
+          if (done[k] || pick[k]) continue;
+          const int g = run[k];
+          const uint64_t m = gbits(g);
+          if (!(m & wait) && __builtin_popcountll(S | m) <= tile_bits && (slot[g] < 0 || ng < AT_MAX_GRADS)) {
+            const int nn = __builtin_popcountll(m & ~S);
+            if (nn < best_new) {
+              best_new = nn;
+              best = (int)k;
+              if (nn == 0) break;
+            }
+          }
+          wait |= m;
+        }
+        if (best < 0) break;
+        pick[best] = 1;
+        ++np;
+        S |= gbits(run[best]);
+        if (slot[run[best]] >= 0) ++ng;
+      }
+      cur.clear();
+      curbits = 0;
+      for (size_t k = 0; k < run.size(); ++k)
+        if (pick[k]) {
+          cur.push_back(run[k]);
+          curbits |= gbits(run[k]);
+          done[k] = 1;
+          --left;
+        }
+      flush_tile();
+    }
+    run.clear();
+  };
   for (int j = ngates - 1; j >= 0; --j) {
+    if (tile && eligible(j)) {
+      run.push_back(j);
+      continue;
+    }
+    if (tile) plan_run();
     if (!eligible(j)) {
       flush();
       if (slot[j] >= 0) {
@@ -400,21 +669,40 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     cur.push_back(j);
     curbits |= b;
   }
+  if (tile) plan_run();
   flush();
   // descriptors after the partial rows, sums and counts
   const uint64_t rows = std::max<size_t>(nb.size(), 1);
   const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * (sizeof(double) + sizeof(int)) + 1024 +
-                        mg.size() * sizeof(MGate) + 256;
+                        mg.size() * sizeof(MGate) + ts.size() * sizeof(TStep) + 512;
   rc = ensure_scratch(ctx, need);
   if (rc) return rc;
   double* partial = (double*)ctx->scratch;
   double* sums = partial + rows * ADJ_MAX_BLOCKS;
   int* d_nb = (int*)(sums + rows);
   MGate* d_mg = (MGate*)(((uintptr_t)(d_nb + rows) + 255) & ~(uintptr_t)255);
+  TStep* d_ts = (TStep*)(((uintptr_t)(d_mg + mg.size()) + 255) & ~(uintptr_t)255);
   if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_mg, mg.data(), mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
+  if (!ts.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_ts, ts.data(), ts.size() * sizeof(TStep), hipMemcpyHostToDevice, s));
   rc = prof_begin(ctx, PROF_GRAD, s);
   if (rc) return rc;
   for (const Item& it : items) {
+    if (it.multi == 2) {
+      const TPass& P = tpasses[it.idx];
+      const uint64_t ntiles = 1ull << (n - P.nb);
+      const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
+      const size_t lds = 2 * ((size_t)1 << P.nb) * sizeof(cx<R>) + AT_MAX_GRADS * (ADJ_THREADS / 64) * sizeof(double);
+      static bool lds_attr = false;  // tiles of 11 bits at complex128 need 65 KiB
+      if (!lds_attr) {
+        QS_CHECK_HIP(hipFuncSetAttribute((const void*)k_adj_tile<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(2 * ((size_t)1 << AT_MAX_BITS) * sizeof(cx<R>) +
+                                               AT_MAX_GRADS * (ADJ_THREADS / 64) * sizeof(double))));
+        lds_attr = true;
+      }
+      hipLaunchKernelGGL(k_adj_tile<R>, dim3(blocks), dim3(ADJ_THREADS), lds, s, psi, lam, ntiles, P, d_ts, partial);
+      QS_CHECK_HIP(hipGetLastError());
+      continue;
+    }
     if (it.multi) {
       const MPass& P = passes[it.idx];
       const uint64_t ngroups = 1ull << (n - P.nb);
