@@ -269,6 +269,7 @@ typedef struct qs_qnn_config {
   int32_t block_threads;
   int32_t blocks_per_cu;
   int32_t use_graph;
+  int32_t fuse_qubits; /* rotations of this many qubits (1..3) per pass over a circuit's state; 0 means 1 */
 } qs_qnn_config;
 
 /* Hardware-efficient-ansatz QNN batch, one persistent launch for forward and

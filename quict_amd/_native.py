@@ -107,7 +107,7 @@ SIGNATURES = {
 
 class QsQnnConfig(ctypes.Structure):
     _fields_ = [("lds_max_width", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("fuse_qubits", ctypes.c_int32)]
 
 _lib = None
 _lib_lock = threading.Lock()

@@ -18,6 +18,13 @@
 // (psi and lam, 2 x 2^w amplitudes) when w <= lds_max_width, otherwise in a
 // per-workgroup global slot.  Workgroups pull circuits from an atomic queue
 // ordered by decreasing width, so the widest circuits start first.
+//
+// Fusion depth D (1..3, a launch parameter the autotuner picks): the rotations
+// of D consecutive qubits of a layer run in one pass over groups of 2^D
+// amplitudes (a coset of the span of their D column masks, enumerated by
+// inserting zeros at the masks' elimination pivots), so a layer costs
+// ceil(w / D) passes forward and backward instead of w: D times fewer sweeps
+// of a wide circuit's state through the memory of its slot.
 #include "qsim_common.hpp"
 #include "qsim_internal.hpp"
 
@@ -33,6 +40,7 @@ constexpr int QNN_MAX_W = 20;
 constexpr int QNN_MAX_LAYERS = 16;
 
 struct QnnArgs {
+  int fuse;               // rotations of `fuse` qubits per pass (1..3)
   int ncirc;
   int nfeat;
   int layers;
@@ -105,6 +113,139 @@ __device__ __forceinline__ void pair_of(uint64_t h, uint64_t m, uint64_t r, uint
   }
 }
 
+// Coset representatives of span{m_0..m_{D-1}}: indices with zeros at the
+// masks' elimination pivots (ascending in piv); returns D.
+template <int D>
+__device__ __forceinline__ void group_pivots(const uint64_t* m, int* piv) {
+  uint64_t red[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    uint64_t v = m[i];
+#pragma unroll
+    for (int j = 0; j < i; ++j)
+      if ((v >> piv[j]) & 1) v ^= red[j];
+    red[i] = v;
+    piv[i] = 63 - __clzll(v);
+  }
+  // ascending order for zero insertion
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i + 1; j < D; ++j)
+      if (piv[j] < piv[i]) {
+        const int t = piv[i];
+        piv[i] = piv[j];
+        piv[j] = t;
+      }
+}
+
+__device__ __forceinline__ uint64_t insert0(uint64_t h, int p) {
+  return ((h >> p) << (p + 1)) | (h & ((1ull << p) - 1));
+}
+
+// Forward: apply U_i (2x2, row-major u[4 i .. 4 i + 3]) on logical qubit i of
+// the group, i < D, to every group of psi.
+template <typename R, int D>
+__device__ __forceinline__ void qnn_fwd_group(cx<R>* psi, uint64_t ngroups, const uint64_t* m, const uint64_t* r,
+                                              const cx<R>* u, int tid, int nt) {
+  constexpr int G = 1 << D;
+  int piv[D];
+  group_pivots<D>(m, piv);
+  uint64_t off[G];
+#pragma unroll
+  for (int sI = 0; sI < G; ++sI) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+      if ((sI >> i) & 1) o ^= m[i];
+    off[sI] = o;
+  }
+  for (uint64_t h = tid; h < ngroups; h += nt) {
+    uint64_t base = h;
+#pragma unroll
+    for (int i = 0; i < D; ++i) base = insert0(base, piv[i]);
+    cx<R> a[G];
+#pragma unroll
+    for (int sI = 0; sI < G; ++sI) a[sI] = ld(psi + (base ^ off[sI]));
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const bool flip = parity64(base & r[i]);  // member with s_i = 1 holds logical 0
+#pragma unroll
+      for (int sI = 0; sI < G; ++sI) {
+        if ((sI >> i) & 1) continue;
+        const int s1 = sI | (1 << i);
+        const cx<R> x0 = flip ? a[s1] : a[sI], x1 = flip ? a[sI] : a[s1];
+        const cx<R> y0 = cfma(mul(u[4 * i], x0), u[4 * i + 1], x1), y1 = cfma(mul(u[4 * i + 2], x0), u[4 * i + 3], x1);
+        a[sI] = flip ? y1 : y0;
+        a[s1] = flip ? y0 : y1;
+      }
+    }
+#pragma unroll
+    for (int sI = 0; sI < G; ++sI) st(psi + (base ^ off[sI]), a[sI]);
+  }
+}
+
+// Backward over one group of D qubits: for each qubit i the Rz step (gradient
+// gz_i = Im <lam|Z|psi>, then Rz^dagger on both) and the Ry step (gy_i =
+// Im <lam|Y|psi>, then Ry^dagger); different qubits commute.
+template <typename R, int D>
+__device__ __forceinline__ void qnn_bwd_group(cx<R>* psi, cx<R>* lam, uint64_t ngroups, const uint64_t* m,
+                                              const uint64_t* r, const double* cy, const double* sy,
+                                              const double* cz, const double* sz, double* gy, double* gz, int tid,
+                                              int nt) {
+  constexpr int G = 1 << D;
+  int piv[D];
+  group_pivots<D>(m, piv);
+  uint64_t off[G];
+#pragma unroll
+  for (int sI = 0; sI < G; ++sI) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+      if ((sI >> i) & 1) o ^= m[i];
+    off[sI] = o;
+  }
+  for (uint64_t h = tid; h < ngroups; h += nt) {
+    uint64_t base = h;
+#pragma unroll
+    for (int i = 0; i < D; ++i) base = insert0(base, piv[i]);
+    cx<R> a[G], b[G];
+#pragma unroll
+    for (int sI = 0; sI < G; ++sI) {
+      a[sI] = ld(psi + (base ^ off[sI]));
+      b[sI] = ld(lam + (base ^ off[sI]));
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const bool flip = parity64(base & r[i]);
+      const cx<R> ep = mk<R>((R)cz[i], (R)sz[i]), em = mk<R>((R)cz[i], (R)-sz[i]);
+#pragma unroll
+      for (int sI = 0; sI < G; ++sI) {
+        if ((sI >> i) & 1) continue;
+        const int s1 = sI | (1 << i);
+        const int j0 = flip ? s1 : sI, j1 = flip ? sI : s1;  // logical 0 / 1 members
+        cx<R> a0 = a[j0], a1 = a[j1], b0 = b[j0], b1 = b[j1];
+        gz[i] += ((double)b0.re * a0.im - (double)b0.im * a0.re) - ((double)b1.re * a1.im - (double)b1.im * a1.re);
+        a0 = mul(ep, a0);
+        b0 = mul(ep, b0);
+        a1 = mul(em, a1);
+        b1 = mul(em, b1);
+        gy[i] += -((double)b0.re * a1.re + (double)b0.im * a1.im) + ((double)b1.re * a0.re + (double)b1.im * a0.im);
+        const R c = (R)cy[i], sn = (R)sy[i];
+        a[j0] = mk<R>(c * a0.re + sn * a1.re, c * a0.im + sn * a1.im);
+        a[j1] = mk<R>(-sn * a0.re + c * a1.re, -sn * a0.im + c * a1.im);
+        b[j0] = mk<R>(c * b0.re + sn * b1.re, c * b0.im + sn * b1.im);
+        b[j1] = mk<R>(-sn * b0.re + c * b1.re, -sn * b0.im + c * b1.im);
+      }
+    }
+#pragma unroll
+    for (int sI = 0; sI < G; ++sI) {
+      st(psi + (base ^ off[sI]), a[sI]);
+      st(lam + (base ^ off[sI]), b[sI]);
+    }
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(1024) void k_qnn(QnnArgs A) {
   extern __shared__ __align__(16) char lds_raw[];
@@ -156,20 +297,25 @@ __global__ __launch_bounds__(1024) void k_qnn(QnnArgs A) {
     __syncthreads();
     // forward
     for (int l = 0; l < A.layers; ++l) {
-      for (int q = 0; q < w; ++q) {
-        const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
-        const double cy = cos(0.5 * ty), sy = sin(0.5 * ty), cz = cos(0.5 * tz), sz = sin(0.5 * tz);
-        // U = Rz(tz) Ry(ty) = [[e^{-i tz/2} cy, -e^{-i tz/2} sy], [e^{i tz/2} sy, e^{i tz/2} cy]]
-        const cx<R> u00 = mk<R>((R)(cz * cy), (R)(-sz * cy)), u01 = mk<R>((R)(-cz * sy), (R)(sz * sy));
-        const cx<R> u10 = mk<R>((R)(cz * sy), (R)(sz * sy)), u11 = mk<R>((R)(cz * cy), (R)(sz * cy));
-        const uint64_t m = cm[q], r = rm[q];
-        for (uint64_t h = tid; h < half; h += nt) {
-          uint64_t p0, p1;
-          pair_of(h, m, r, p0, p1);
-          const cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1);
-          st(psi + p0, cfma(mul(u00, a0), u01, a1));
-          st(psi + p1, cfma(mul(u10, a0), u11, a1));
+      for (int q0 = 0; q0 < w; q0 += A.fuse) {
+        const int d = min(A.fuse, w - q0);
+        uint64_t m[3], r[3];
+        cx<R> u[12];
+        for (int i = 0; i < d; ++i) {
+          const int q = q0 + i;
+          const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
+          const double cy = cos(0.5 * ty), sy = sin(0.5 * ty), cz = cos(0.5 * tz), sz = sin(0.5 * tz);
+          // U = Rz(tz) Ry(ty) = [[e^{-i tz/2} cy, -e^{-i tz/2} sy], [e^{i tz/2} sy, e^{i tz/2} cy]]
+          u[4 * i] = mk<R>((R)(cz * cy), (R)(-sz * cy));
+          u[4 * i + 1] = mk<R>((R)(-cz * sy), (R)(sz * sy));
+          u[4 * i + 2] = mk<R>((R)(cz * sy), (R)(sz * sy));
+          u[4 * i + 3] = mk<R>((R)(cz * cy), (R)(sz * cy));
+          m[i] = cm[q];
+          r[i] = rm[q];
         }
+        if (d == 1) qnn_fwd_group<R, 1>(psi, N >> 1, m, r, u, tid, nt);
+        else if (d == 2) qnn_fwd_group<R, 2>(psi, N >> 2, m, r, u, tid, nt);
+        else qnn_fwd_group<R, 3>(psi, N >> 3, m, r, u, tid, nt);
         __syncthreads();
       }
       // CX ladder: r'_k = r_0 ^ .. ^ r_k ; c'_q = c_q ^ c_{q+1}
@@ -201,40 +347,31 @@ __global__ __launch_bounds__(1024) void k_qnn(QnnArgs A) {
         for (int q = w - 2; q >= 0; --q) cm[q] ^= cm[q + 1];
       }
       __syncthreads();
-      for (int q = 0; q < w; ++q) {
-        const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
-        const double cy = cos(0.5 * ty), sy = sin(0.5 * ty), cz = cos(0.5 * tz), sz = sin(0.5 * tz);
-        const uint64_t m = cm[q], r = rm[q];
-        double gz = 0, gy = 0;
-        for (uint64_t h = tid; h < half; h += nt) {
-          uint64_t p0, p1;
-          pair_of(h, m, r, p0, p1);
-          cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1), b0 = ld(lam + p0), b1 = ld(lam + p1);
-          // d/dtz: Im <lam| Z |psi>
-          gz += ((double)b0.re * a0.im - (double)b0.im * a0.re) - ((double)b1.re * a1.im - (double)b1.im * a1.re);
-          // Rz^dagger = diag(e^{i tz/2}, e^{-i tz/2})
-          const cx<R> ep = mk<R>((R)cz, (R)sz), em = mk<R>((R)cz, (R)-sz);
-          a0 = mul(ep, a0);
-          b0 = mul(ep, b0);
-          a1 = mul(em, a1);
-          b1 = mul(em, b1);
-          // d/dty: Im <lam| Y |psi> with Y psi = (-i a1, i a0):
-          // Im(conj(b0)(-i a1)) = -Re(conj(b0) a1),  Im(conj(b1)(i a0)) = Re(conj(b1) a0)
-          gy += -((double)b0.re * a1.re + (double)b0.im * a1.im) + ((double)b1.re * a0.re + (double)b1.im * a0.im);
-          // Ry^dagger = [[cy, sy], [-sy, cy]]
-          const cx<R> n0 = mk<R>((R)(cy * a0.re + sy * a1.re), (R)(cy * a0.im + sy * a1.im));
-          const cx<R> n1 = mk<R>((R)(-sy * a0.re + cy * a1.re), (R)(-sy * a0.im + cy * a1.im));
-          const cx<R> l0 = mk<R>((R)(cy * b0.re + sy * b1.re), (R)(cy * b0.im + sy * b1.im));
-          const cx<R> l1 = mk<R>((R)(-sy * b0.re + cy * b1.re), (R)(-sy * b0.im + cy * b1.im));
-          st(psi + p0, n0);
-          st(psi + p1, n1);
-          st(lam + p0, l0);
-          st(lam + p1, l1);
+      for (int q0 = 0; q0 < w; q0 += A.fuse) {
+        const int d = min(A.fuse, w - q0);
+        uint64_t m[3], r[3];
+        double cy[3], sy[3], cz[3], sz[3], gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+        for (int i = 0; i < d; ++i) {
+          const int q = q0 + i;
+          const double ty = th[(l * 2 + 0) * MW + q], tz = th[(l * 2 + 1) * MW + q];
+          cy[i] = cos(0.5 * ty);
+          sy[i] = sin(0.5 * ty);
+          cz[i] = cos(0.5 * tz);
+          sz[i] = sin(0.5 * tz);
+          m[i] = cm[q];
+          r[i] = rm[q];
         }
-        block_reduce2<R>(gz, gy, red);
-        if (tid == 0) {
-          gacc[(l * 2 + 0) * MW + q] += gy;
-          gacc[(l * 2 + 1) * MW + q] += gz;
+        // per qubit: gz = Im <lam|Z|psi>, Rz^dagger = diag(e^{i tz/2}, e^{-i tz/2}) on both,
+        // gy = Im <lam|Y|psi> (Y psi = (-i a1, i a0)), Ry^dagger = [[cy, sy], [-sy, cy]] on both
+        if (d == 1) qnn_bwd_group<R, 1>(psi, lam, N >> 1, m, r, cy, sy, cz, sz, gy, gz, tid, nt);
+        else if (d == 2) qnn_bwd_group<R, 2>(psi, lam, N >> 2, m, r, cy, sy, cz, sz, gy, gz, tid, nt);
+        else qnn_bwd_group<R, 3>(psi, lam, N >> 3, m, r, cy, sy, cz, sz, gy, gz, tid, nt);
+        for (int i = 0; i < d; ++i) {
+          block_reduce2<R>(gz[i], gy[i], red);
+          if (tid == 0) {
+            gacc[(l * 2 + 0) * MW + q0 + i] += gy[i];
+            gacc[(l * 2 + 1) * MW + q0 + i] += gz[i];
+          }
         }
         __syncthreads();
       }
@@ -283,7 +420,8 @@ __global__ void k_qnn_reduce(const double* __restrict__ grad_circ, int ncirc, in
 }
 
 struct QnnPlan {
-  int dtype = -1, ncirc = 0, nfeat = 0, layers = 0, max_width = 0, lds_max_width = 0, threads = 0, blocks = 0;
+  int dtype = -1, ncirc = 0, nfeat = 0, layers = 0, max_width = 0, lds_max_width = 0, threads = 0, blocks = 0,
+      fuse = 0;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t stream = nullptr;
@@ -301,6 +439,8 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   int threads = cfg ? cfg->block_threads : 256;
   int per_cu = cfg ? cfg->blocks_per_cu : 2;
   const int use_graph = cfg ? cfg->use_graph : 1;
+  const int fuse = (cfg && cfg->fuse_qubits) ? cfg->fuse_qubits : 1;
+  QS_REQUIRE(fuse >= 1 && fuse <= 3, "fuse_qubits must be in [1, 3] (0: 1)");
   QS_REQUIRE(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
              "block_threads must be a power of two in [64, 1024]");
   QS_REQUIRE(lds_w >= 0 && lds_w <= 13, "lds_max_width must be in [0, 13]");
@@ -322,7 +462,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   QnnPlan& P = g_plan;
   const bool same = P.exec && P.dtype == dtype && P.ncirc == ncirc && P.nfeat == nfeat && P.layers == layers &&
                     P.max_width == max_width && P.lds_max_width == lds_w && P.threads == threads &&
-                    P.blocks == blocks;
+                    P.blocks == blocks && P.fuse == fuse;
   if (!same) {
     if (P.exec) (void)hipGraphExecDestroy(P.exec);
     if (P.graph) (void)hipGraphDestroy(P.graph);
@@ -362,6 +502,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   QS_CHECK_HIP(hipMemcpyAsync(d_th, theta, nth * sizeof(double), hipMemcpyHostToDevice, s));
 
   QnnArgs A;
+  A.fuse = fuse;
   A.ncirc = ncirc;
   A.nfeat = nfeat;
   A.layers = layers;
@@ -416,6 +557,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   P.lds_max_width = lds_w;
   P.threads = threads;
   P.blocks = blocks;
+  P.fuse = fuse;
   std::vector<double> lt(ncirc);
   QS_CHECK_HIP(hipMemcpyAsync(expval, d_e, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
   QS_CHECK_HIP(hipMemcpyAsync(lt.data(), d_l, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -26,7 +26,7 @@ from .adjoint import ParameterizedCircuit
 CACHE_PATH = os.environ.get("QUICT_AMD_TUNE_CACHE",
                             os.path.join(os.path.expanduser("~"), ".cache", "quict_amd", "autotune.json"))
 
-DEFAULT_CONFIG = {"lds_max_width": 11, "block_threads": 256, "blocks_per_cu": 2, "use_graph": 1}
+DEFAULT_CONFIG = {"lds_max_width": 11, "block_threads": 256, "blocks_per_cu": 2, "use_graph": 1, "fuse_qubits": 2}
 
 
 def circuit_for(width: int, x_row: np.ndarray, layers: int, max_width: int) -> ParameterizedCircuit:
@@ -83,6 +83,7 @@ class QNNBatch:
         c.block_threads = cfg["block_threads"]
         c.blocks_per_cu = cfg["blocks_per_cu"]
         c.use_graph = cfg.get("use_graph", 1)
+        c.fuse_qubits = cfg.get("fuse_qubits", 1)
         return c
 
     def forward_backward(self, theta: np.ndarray, config: Optional[Dict] = None
@@ -112,14 +113,18 @@ class QNNBatch:
         return json.dumps({"dtype": self.precision, "hist": hist, "layers": self.layers, "nfeat": self.nfeat})
 
     def search_space(self):
+        """LDS width limit (tile size) x fusion depth (qubits per pass) x workgroup size x
+        workgroups per CU."""
         lds_cap = 12 if self.precision == "double" else 13
         for lds in sorted({min(10, lds_cap), min(11, lds_cap), lds_cap}):
-            for threads in (128, 256, 512):
-                for per_cu in (1, 2, 4):
-                    amp = 16 if self.precision == "double" else 8
-                    if per_cu * 2 * (1 << lds) * amp > 160 * 1024:
-                        continue
-                    yield {"lds_max_width": lds, "block_threads": threads, "blocks_per_cu": per_cu, "use_graph": 1}
+            for fuse in (1, 2, 3):
+                for threads in (256, 512):
+                    for per_cu in (1, 2):
+                        amp = 16 if self.precision == "double" else 8
+                        if per_cu * 2 * (1 << lds) * amp > 160 * 1024:
+                            continue
+                        yield {"lds_max_width": lds, "block_threads": threads, "blocks_per_cu": per_cu,
+                               "use_graph": 1, "fuse_qubits": fuse}
 
     def autotune(self, theta: np.ndarray, reps: int = 2, use_cache: bool = True) -> Dict:
         key = self._key()

@@ -159,3 +159,49 @@ def test_batch_gradient_is_bit_reproducible_across_launch_shapes():
         assert loss == outs[0][0]
         assert np.array_equal(ev, outs[0][1])
         assert np.array_equal(g, outs[0][2])
+
+
+@pytest.mark.parametrize("fuse", [1, 2, 3])
+def test_fusion_depth_vs_cpu_shift(fuse):
+    """Rotations of 1, 2 or 3 qubits per pass (odd widths leave a shorter last
+    group): the small batch against the CPU oracle's parameter shift, LDS-resident
+    and global-slot states both."""
+    say("qnn fusion depth", fuse)
+    from quict_amd.gradient.qnn import QNNBatch
+    widths = np.array([3, 5, 4, 7, 2, 6, 5], dtype=np.int32)
+    rng = np.random.RandomState(12)
+    x = rng.randn(7, 9)
+    t = 1 - 2 * (x[:, 0] > 0)
+    layers, mw = 3, 7
+    theta = rng.uniform(-np.pi, np.pi, layers * 2 * mw)
+    rl, rev, rg = reference(widths, x, t, theta, layers, mw)
+    for lds in (4, 7):  # widths above `lds` live in the global slot
+        qb = QNNBatch(widths, x, t, layers=layers, max_width=mw,
+                      config={"lds_max_width": lds, "block_threads": 128, "blocks_per_cu": 1, "use_graph": 1,
+                              "fuse_qubits": fuse})
+        loss, ev, gth, gx = qb.forward_backward(theta)
+        assert np.max(np.abs(ev - rev)) < 1e-12
+        assert abs(loss - rl) < 1e-12
+        assert np.max(np.abs(gth - rg)) < 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-11), ("single", 2e-4)])
+def test_fusion_depths_agree_on_config_batch(precision, tol):
+    """The 1024-circuit configuration batch (widths 8..18): fusion depths 2 and 3
+    give the loss, <Z_0> and gradient of depth 1."""
+    say("qnn fusion depths agree", precision)
+    from quict_amd.gradient.qnn import QNNBatch, synthetic_batch
+    widths, x, t = synthetic_batch(1024, 16, 8, 18, 4)
+    layers, mw = 6, 18
+    theta = np.random.RandomState(7).uniform(-np.pi, np.pi, layers * 2 * mw)
+    outs = {}
+    for fuse in (1, 2, 3):
+        qb = QNNBatch(widths, x, t, layers=layers, max_width=mw, precision=precision,
+                      config={"lds_max_width": 11, "block_threads": 256, "blocks_per_cu": 2, "use_graph": 1,
+                              "fuse_qubits": fuse})
+        outs[fuse] = qb.forward_backward(theta)
+        print(f"  fuse {fuse}: {qb.last_ms:.1f} ms")
+    for fuse in (2, 3):
+        assert abs(outs[fuse][0] - outs[1][0]) < tol
+        assert np.max(np.abs(outs[fuse][1] - outs[1][1])) < tol
+        assert np.max(np.abs(outs[fuse][2] - outs[1][2])) < tol
