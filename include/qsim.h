@@ -286,6 +286,41 @@ int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
                  const double* target, int layers, int max_width, const double* theta, double* expval,
                  double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg, void* stream);
 
+/* One op of a per-circuit QNN program (no reference counterpart: QuICT's QNN
+   code lives in the external QuICT_ml package).  kind: QS_QNN_RX / _RY / _RZ,
+   exp(-i t P / 2) on qubit q0 with t = theta[param] (trainable) or, when
+   param < 0, t = value; QS_QNN_ENC_RY, Ry(x[c*nfeat + param]) on q0 (an input
+   encoding: its gradient goes to grad_x); QS_QNN_CX, control q0, target q1;
+   QS_QNN_CZ on q0, q1.  Qubit 0 is the most significant amplitude bit. */
+#define QS_QNN_RX 0
+#define QS_QNN_RY 1
+#define QS_QNN_RZ 2
+#define QS_QNN_CX 3
+#define QS_QNN_CZ 4
+#define QS_QNN_ENC_RY 5
+typedef struct qs_qnn_op {
+  int32_t kind;
+  int32_t q0;
+  int32_t q1;
+  int32_t param;
+  double value;
+} qs_qnn_op;
+
+/* Batch of circuits with their own gate programs, forward and adjoint backward
+   in one persistent launch (no reference counterpart).  Circuit c has width
+   widths[c] (1..20) and runs ops[prog_offsets[c] .. prog_offsets[c+1]) from
+   |0..0>; E_c = <Z_0>, loss = mean over circuits of (E_c - target[c])^2.
+   theta (nparams values) is shared by every circuit.  Outputs (host):
+   expval[c], loss, grad_theta (d loss / d theta, summed over circuits in
+   circuit order: bit-identical across runs), grad_x (ncirc x nfeat, d loss /
+   d x for the encoding ops; may be NULL).  cfg: lds_max_width, block_threads,
+   blocks_per_cu (use_graph and fuse_qubits are ignored: one rotation per
+   pass); NULL for defaults.  Replaces no reference function. */
+int qs_qnn_batch_programs(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, const int32_t* prog_offsets,
+                          const qs_qnn_op* ops, int nparams, int nfeat, const double* x, const double* target,
+                          const double* theta, double* expval, double* loss, double* grad_theta, double* grad_x,
+                          const qs_qnn_config* cfg, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

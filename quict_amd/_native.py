@@ -102,12 +102,20 @@ SIGNATURES = {
     "qs_sv_inner": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_adjoint_gradient": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _I, _P]),
     "qs_qnn_batch": (_I, [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "qs_qnn_batch_programs": (_I, [_P, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 
 class QsQnnConfig(ctypes.Structure):
     _fields_ = [("lds_max_width", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("fuse_qubits", ctypes.c_int32)]
+
+class QsQnnOp(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("q0", ctypes.c_int32), ("q1", ctypes.c_int32),
+                ("param", ctypes.c_int32), ("value", ctypes.c_double)]
+
+
+QS_QNN_RX, QS_QNN_RY, QS_QNN_RZ, QS_QNN_CX, QS_QNN_CZ, QS_QNN_ENC_RY = range(6)
 
 _lib = None
 _lib_lock = threading.Lock()

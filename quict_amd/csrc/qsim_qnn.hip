@@ -572,6 +572,315 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   return QS_OK;
 }
 
+// ---- per-circuit gate programs --------------------------------------------------
+// Each circuit runs its own op list (qs_qnn_op): Rx / Ry / Rz on a logical qubit
+// with a shared trainable angle theta[param] or a fixed value, Ry(x[c, j])
+// encodings, CX (a GF(2) update of the qubit masks, no amplitude moves) and CZ
+// (one sign pass).  Forward, E_c = <Z_0>, lam = Z_0 psi, then the ops in reverse:
+// a trainable rotation exp(-i t P / 2) contributes Im <lam|P|psi> (both at the
+// point after the gate) to d E / d t before its inverse is applied to both;
+// an encoding contributes the same to d E / d x[c, j].  One rotation per pass.
+struct QnnProgArgs {
+  int ncirc, nfeat, nparams, lds_max_width, max_width;
+  const int32_t* order;   // circuits, widest first
+  const int32_t* widths;
+  const int32_t* off;     // ncirc + 1 op offsets
+  const qs_qnn_op* ops;
+  const double* x;        // ncirc x nfeat
+  const double* target;
+  const double* theta;    // nparams
+  double* expval;
+  double* loss_terms;
+  double* grad_circ;      // ncirc x nparams
+  double* grad_x;         // ncirc x nfeat
+  void* slots;
+  int* counter;
+  double inv_ncirc;
+};
+
+// 2x2 rotation exp(-i t P / 2), P = X, Y, Z (kinds QS_QNN_RX, RY / ENC_RY, RZ)
+template <typename R> __device__ __forceinline__ void rot_matrix(int axis, double t, cx<R>* u) {
+  const double c = cos(0.5 * t), s = sin(0.5 * t);
+  if (axis == 0) {  // X: [[c, -i s], [-i s, c]]
+    u[0] = mk<R>((R)c, (R)0);
+    u[1] = mk<R>((R)0, (R)-s);
+    u[2] = mk<R>((R)0, (R)-s);
+    u[3] = mk<R>((R)c, (R)0);
+  } else if (axis == 1) {  // Y: [[c, -s], [s, c]]
+    u[0] = mk<R>((R)c, (R)0);
+    u[1] = mk<R>((R)-s, (R)0);
+    u[2] = mk<R>((R)s, (R)0);
+    u[3] = mk<R>((R)c, (R)0);
+  } else {  // Z: diag(e^{-i t/2}, e^{i t/2})
+    u[0] = mk<R>((R)c, (R)-s);
+    u[1] = mk<R>((R)0, (R)0);
+    u[2] = mk<R>((R)0, (R)0);
+    u[3] = mk<R>((R)c, (R)s);
+  }
+}
+
+__device__ __forceinline__ int op_axis(int kind) { return kind == QS_QNN_RX ? 0 : (kind == QS_QNN_RZ ? 2 : 1); }
+
+template <typename R>
+__global__ __launch_bounds__(1024) void k_qnn_prog(QnnProgArgs A) {
+  extern __shared__ __align__(16) char lds_raw[];
+  __shared__ double red[64];
+  __shared__ uint64_t cm[QNN_MAX_W], rm[QNN_MAX_W];
+  __shared__ int cur;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  cx<R>* slot_psi = reinterpret_cast<cx<R>*>(A.slots) + (uint64_t)blockIdx.x * 2 * (1ull << A.max_width);
+  cx<R>* slot_lam = slot_psi + (1ull << A.max_width);
+  for (;;) {
+    if (tid == 0) cur = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int k = cur;
+    __syncthreads();
+    if (k >= A.ncirc) break;
+    const int c = A.order[k];
+    const int w = A.widths[c];
+    const uint64_t N = 1ull << w, half = N >> 1;
+    cx<R>* psi;
+    cx<R>* lam;
+    if (w <= A.lds_max_width) {
+      psi = reinterpret_cast<cx<R>*>(lds_raw);
+      lam = psi + N;
+    } else {
+      psi = slot_psi;
+      lam = slot_lam;
+    }
+    const qs_qnn_op* ops = A.ops + A.off[c];
+    const int nops = A.off[c + 1] - A.off[c];
+    if (tid < w) cm[tid] = rm[tid] = 1ull << (w - 1 - tid);
+    for (uint64_t i = tid; i < N; i += nt) st(psi + i, mk<R>((R)(i == 0 ? 1 : 0), (R)0));
+    __syncthreads();
+    // forward
+    for (int o = 0; o < nops; ++o) {
+      const qs_qnn_op op = ops[o];
+      if (op.kind == QS_QNN_CX) {
+        if (tid == 0) {
+          rm[op.q1] ^= rm[op.q0];
+          cm[op.q0] ^= cm[op.q1];
+        }
+      } else if (op.kind == QS_QNN_CZ) {
+        const uint64_t ra = rm[op.q0], rb = rm[op.q1];
+        for (uint64_t i = tid; i < N; i += nt)
+          if (parity64(i & ra) & parity64(i & rb)) {
+            const cx<R> a = ld(psi + i);
+            st(psi + i, mk<R>(-a.re, -a.im));
+          }
+      } else {
+        const double t = op.kind == QS_QNN_ENC_RY ? A.x[(uint64_t)c * A.nfeat + op.param]
+                                                  : (op.param >= 0 ? A.theta[op.param] : op.value);
+        cx<R> u[4];
+        rot_matrix<R>(op_axis(op.kind), t, u);
+        const uint64_t m = cm[op.q0], r = rm[op.q0];
+        for (uint64_t h = tid; h < half; h += nt) {
+          uint64_t p0, p1;
+          pair_of(h, m, r, p0, p1);
+          const cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1);
+          st(psi + p0, cfma(mul(u[0], a0), u[1], a1));
+          st(psi + p1, cfma(mul(u[2], a0), u[3], a1));
+        }
+      }
+      __syncthreads();
+    }
+    double e = 0;
+    {
+      const uint64_t r0 = rm[0];
+      for (uint64_t i = tid; i < N; i += nt) {
+        const cx<R> a = ld(psi + i);
+        const double pr = (double)a.re * a.re + (double)a.im * a.im;
+        const bool one = parity64(i & r0);
+        e += one ? -pr : pr;
+        st(lam + i, one ? mk<R>(-a.re, -a.im) : a);
+      }
+    }
+    e = block_reduce<R>(e, red);
+    const double tgt = A.target[c];
+    const double dl = 2.0 * (e - tgt) * A.inv_ncirc;
+    double* grow = A.grad_circ + (uint64_t)c * A.nparams;
+    for (int i = tid; i < A.nparams; i += nt) grow[i] = 0.0;
+    for (int j = tid; j < A.nfeat; j += nt) A.grad_x[(uint64_t)c * A.nfeat + j] = 0.0;
+    __syncthreads();
+    // backward
+    for (int o = nops - 1; o >= 0; --o) {
+      const qs_qnn_op op = ops[o];
+      if (op.kind == QS_QNN_CX) {
+        if (tid == 0) {
+          rm[op.q1] ^= rm[op.q0];
+          cm[op.q0] ^= cm[op.q1];
+        }
+      } else if (op.kind == QS_QNN_CZ) {
+        const uint64_t ra = rm[op.q0], rb = rm[op.q1];
+        for (uint64_t i = tid; i < N; i += nt)
+          if (parity64(i & ra) & parity64(i & rb)) {
+            const cx<R> a = ld(psi + i), b = ld(lam + i);
+            st(psi + i, mk<R>(-a.re, -a.im));
+            st(lam + i, mk<R>(-b.re, -b.im));
+          }
+      } else {
+        const bool enc = op.kind == QS_QNN_ENC_RY;
+        const bool train = enc || op.param >= 0;
+        const int axis = op_axis(op.kind);
+        const double t = enc ? A.x[(uint64_t)c * A.nfeat + op.param] : (op.param >= 0 ? A.theta[op.param] : op.value);
+        cx<R> u[4];
+        rot_matrix<R>(axis, -t, u);  // the inverse
+        const uint64_t m = cm[op.q0], r = rm[op.q0];
+        double g = 0;
+        for (uint64_t h = tid; h < half; h += nt) {
+          uint64_t p0, p1;
+          pair_of(h, m, r, p0, p1);
+          const cx<R> a0 = ld(psi + p0), a1 = ld(psi + p1), b0 = ld(lam + p0), b1 = ld(lam + p1);
+          if (train) {
+            // Im <lam|P|psi>: X psi = (a1, a0); Y psi = (-i a1, i a0); Z psi = (a0, -a1)
+            if (axis == 0)
+              g += ((double)b0.re * a1.im - (double)b0.im * a1.re) + ((double)b1.re * a0.im - (double)b1.im * a0.re);
+            else if (axis == 1)
+              g += -((double)b0.re * a1.re + (double)b0.im * a1.im) + ((double)b1.re * a0.re + (double)b1.im * a0.im);
+            else
+              g += ((double)b0.re * a0.im - (double)b0.im * a0.re) - ((double)b1.re * a1.im - (double)b1.im * a1.re);
+          }
+          st(psi + p0, cfma(mul(u[0], a0), u[1], a1));
+          st(psi + p1, cfma(mul(u[2], a0), u[3], a1));
+          st(lam + p0, cfma(mul(u[0], b0), u[1], b1));
+          st(lam + p1, cfma(mul(u[2], b0), u[3], b1));
+        }
+        if (train) {
+          double dummy = 0;
+          block_reduce2<R>(g, dummy, red);
+          if (tid == 0) {
+            if (enc) A.grad_x[(uint64_t)c * A.nfeat + op.param] += dl * g;
+            else grow[op.param] += dl * g;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      A.expval[c] = e;
+      A.loss_terms[c] = (e - tgt) * (e - tgt);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R>
+static int qnn_prog_t(qs_ctx* ctx, int ncirc, const int32_t* widths, const int32_t* off, const qs_qnn_op* ops,
+                      int nparams, int nfeat, const double* x, const double* target, const double* theta,
+                      double* expval, double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg,
+                      hipStream_t s) {
+  const int lds_w = cfg ? cfg->lds_max_width : 11;
+  const int threads = cfg ? cfg->block_threads : 256;
+  const int per_cu = cfg ? cfg->blocks_per_cu : 2;
+  QS_REQUIRE(threads == 64 || threads == 128 || threads == 256 || threads == 512 || threads == 1024,
+             "block_threads must be a power of two in [64, 1024]");
+  QS_REQUIRE(lds_w >= 0 && lds_w <= 13, "lds_max_width must be in [0, 13]");
+  QS_REQUIRE(per_cu >= 1 && per_cu <= 8, "blocks_per_cu must be in [1, 8]");
+  const size_t lds_bytes = lds_w > 0 ? (size_t)2 * (1ull << lds_w) * sizeof(cx<R>) : 0;
+  QS_REQUIRE(lds_bytes <= 150 * 1024, "LDS state too large for one workgroup");
+  int max_width = 1;
+  for (int i = 0; i < ncirc; ++i) max_width = std::max(max_width, (int)widths[i]);
+  int cus = 0;
+  QS_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int blocks = std::max(1, cus * per_cu);
+  const int nops = off[ncirc];
+  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+  const uint64_t np = std::max(nparams, 1), nf = std::max(nfeat, 1);
+  const uint64_t sz_i = al((uint64_t)ncirc * sizeof(int32_t)), sz_o = al((uint64_t)(ncirc + 1) * sizeof(int32_t)),
+                 sz_ops = al((uint64_t)std::max(nops, 1) * sizeof(qs_qnn_op)),
+                 sz_x = al((uint64_t)ncirc * nf * sizeof(double)), sz_c = al((uint64_t)ncirc * sizeof(double)),
+                 sz_t = al(np * sizeof(double)), sz_gc = al((uint64_t)ncirc * np * sizeof(double));
+  const uint64_t slot_bytes = (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>);
+  const uint64_t total = 2 * sz_i + sz_o + sz_ops + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + sz_gc + slot_bytes;
+  int rc = ensure_scratch(ctx, total);
+  if (rc) return rc;
+  char* b = (char*)ctx->scratch;
+  int32_t* d_order = (int32_t*)b;
+  b += sz_i;
+  int32_t* d_w = (int32_t*)b;
+  b += sz_i;
+  int32_t* d_off = (int32_t*)b;
+  b += sz_o;
+  qs_qnn_op* d_ops = (qs_qnn_op*)b;
+  b += sz_ops;
+  double* d_x = (double*)b;
+  b += sz_x;
+  double* d_gx = (double*)b;
+  b += sz_x;
+  double* d_t = (double*)b;
+  b += sz_c;
+  double* d_e = (double*)b;
+  b += sz_c;
+  double* d_l = (double*)b;
+  b += sz_c;
+  double* d_th = (double*)b;
+  b += sz_t;
+  double* d_gth = (double*)b;
+  b += sz_t;
+  int* d_cnt = (int*)b;
+  b += 256;
+  double* d_gc = (double*)b;
+  b += sz_gc;
+  void* d_slots = b;
+  std::vector<int32_t> order(ncirc);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c2) { return widths[a] > widths[c2]; });
+  QS_CHECK_HIP(hipMemcpyAsync(d_order, order.data(), ncirc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_w, widths, ncirc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_off, off, (ncirc + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (nops) QS_CHECK_HIP(hipMemcpyAsync(d_ops, ops, nops * sizeof(qs_qnn_op), hipMemcpyHostToDevice, s));
+  if (nfeat) QS_CHECK_HIP(hipMemcpyAsync(d_x, x, (uint64_t)ncirc * nfeat * sizeof(double), hipMemcpyHostToDevice, s));
+  QS_CHECK_HIP(hipMemcpyAsync(d_t, target, ncirc * sizeof(double), hipMemcpyHostToDevice, s));
+  if (nparams) QS_CHECK_HIP(hipMemcpyAsync(d_th, theta, nparams * sizeof(double), hipMemcpyHostToDevice, s));
+  QnnProgArgs A;
+  A.ncirc = ncirc;
+  A.nfeat = nfeat;
+  A.nparams = nparams;
+  A.lds_max_width = lds_w;
+  A.max_width = max_width;
+  A.order = d_order;
+  A.widths = d_w;
+  A.off = d_off;
+  A.ops = d_ops;
+  A.x = d_x;
+  A.target = d_t;
+  A.theta = d_th;
+  A.expval = d_e;
+  A.loss_terms = d_l;
+  A.grad_circ = d_gc;
+  A.grad_x = d_gx;
+  A.slots = d_slots;
+  A.counter = d_cnt;
+  A.inv_ncirc = 1.0 / ncirc;
+  if (lds_bytes > 64 * 1024)
+    QS_CHECK_HIP(hipFuncSetAttribute((const void*)k_qnn_prog<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds_bytes));
+  rc = prof_begin(ctx, PROF_QNN, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_qnn_clear, dim3(1), dim3(64), 0, s, d_cnt);
+  QS_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_qnn_prog<R>, dim3(blocks), dim3(threads), lds_bytes, s, A);
+  QS_CHECK_HIP(hipGetLastError());
+  if (nparams) {
+    hipLaunchKernelGGL(k_qnn_reduce, dim3((unsigned)((nparams + 255) / 256)), dim3(256), 0, s, d_gc, ncirc, nparams,
+                       d_gth);
+    QS_CHECK_HIP(hipGetLastError());
+  }
+  rc = prof_end(ctx, s);
+  if (rc) return rc;
+  std::vector<double> lt(ncirc);
+  QS_CHECK_HIP(hipMemcpyAsync(expval, d_e, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
+  QS_CHECK_HIP(hipMemcpyAsync(lt.data(), d_l, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (nparams) QS_CHECK_HIP(hipMemcpyAsync(grad_theta, d_gth, nparams * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (grad_x && nfeat)
+    QS_CHECK_HIP(hipMemcpyAsync(grad_x, d_gx, (uint64_t)ncirc * nfeat * sizeof(double), hipMemcpyDeviceToHost, s));
+  QS_CHECK_HIP(hipStreamSynchronize(s));
+  double L = 0;
+  for (int i = 0; i < ncirc; ++i) L += lt[i];
+  *loss = L / ncirc;
+  return QS_OK;
+}
+
 }  // namespace qs
 
 using namespace qs;
@@ -590,4 +899,37 @@ extern "C" int qs_qnn_batch(qs_ctx* ctx, int dtype, int ncirc, const int32_t* wi
                          grad_theta, grad_x, cfg, as_stream(stream));
   return qnn_t<float>(ctx, dtype, ncirc, widths, nfeat, x, target, layers, max_width, theta, expval, loss, grad_theta,
                       grad_x, cfg, as_stream(stream));
+}
+
+extern "C" int qs_qnn_batch_programs(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths,
+                                     const int32_t* prog_offsets, const qs_qnn_op* ops, int nparams, int nfeat,
+                                     const double* x, const double* target, const double* theta, double* expval,
+                                     double* loss, double* grad_theta, double* grad_x, const qs_qnn_config* cfg,
+                                     void* stream) {
+  QS_REQUIRE(ctx && widths && prog_offsets && target && expval && loss, "null argument");
+  QS_REQUIRE(dtype == QS_C64 || dtype == QS_C128, "dtype must be QS_C64 or QS_C128");
+  QS_REQUIRE(ncirc >= 1 && nparams >= 0 && nfeat >= 0, "bad batch shape");
+  QS_REQUIRE(nparams == 0 || (theta && grad_theta), "theta / grad_theta missing");
+  QS_REQUIRE(nfeat == 0 || x, "x missing");
+  QS_REQUIRE(prog_offsets[0] == 0, "prog_offsets[0] must be 0");
+  for (int c = 0; c < ncirc; ++c) {
+    const int w = widths[c];
+    QS_REQUIRE(w >= 1 && w <= QNN_MAX_W, "width out of range [1, 20]");
+    QS_REQUIRE(prog_offsets[c + 1] >= prog_offsets[c], "prog_offsets must not decrease");
+    QS_REQUIRE(prog_offsets[c + 1] == prog_offsets[c] || ops, "ops missing");
+    for (int o = prog_offsets[c]; o < prog_offsets[c + 1]; ++o) {
+      const qs_qnn_op& op = ops[o];
+      QS_REQUIRE(op.kind >= QS_QNN_RX && op.kind <= QS_QNN_ENC_RY, "unknown op kind");
+      QS_REQUIRE(op.q0 >= 0 && op.q0 < w, "op qubit out of range");
+      if (op.kind == QS_QNN_CX || op.kind == QS_QNN_CZ)
+        QS_REQUIRE(op.q1 >= 0 && op.q1 < w && op.q1 != op.q0, "two-qubit op needs two distinct qubits");
+      if (op.kind == QS_QNN_ENC_RY) QS_REQUIRE(op.param >= 0 && op.param < nfeat, "encoding feature out of range");
+      else if (op.kind <= QS_QNN_RZ) QS_REQUIRE(op.param < nparams, "parameter index out of range");
+    }
+  }
+  if (dtype == QS_C128)
+    return qnn_prog_t<double>(ctx, ncirc, widths, prog_offsets, ops, nparams, nfeat, x, target, theta, expval, loss,
+                              grad_theta, grad_x, cfg, as_stream(stream));
+  return qnn_prog_t<float>(ctx, ncirc, widths, prog_offsets, ops, nparams, nfeat, x, target, theta, expval, loss,
+                           grad_theta, grad_x, cfg, as_stream(stream));
 }

@@ -157,3 +157,85 @@ class QNNBatch:
                 json.dump(cache, f)
         self.tuning_trials = trials
         return best
+
+
+_PROGRAM_KINDS = {GateType.rx: nat.QS_QNN_RX, GateType.ry: nat.QS_QNN_RY, GateType.rz: nat.QS_QNN_RZ}
+
+
+def program_ops(pc: ParameterizedCircuit, encode=()) -> list:
+    """A circuit as qs_qnn_op records: Ry(x[feature]) encodings on the given
+    (qubit, feature) pairs first, then the circuit's Rx / Ry / Rz (trainable
+    parameter or fixed angle), CX and CZ gates."""
+    ops = []
+    for q, j in encode:
+        ops.append((nat.QS_QNN_ENC_RY, int(q), 0, int(j), 0.0))
+    for gtype, qubits, param, vals in pc.ops:
+        if gtype in _PROGRAM_KINDS:
+            if param is not None:
+                ops.append((_PROGRAM_KINDS[gtype], qubits[0], 0, int(param), 0.0))
+            else:
+                ops.append((_PROGRAM_KINDS[gtype], qubits[0], 0, -1, float(vals[0])))
+        elif gtype == GateType.cx:
+            ops.append((nat.QS_QNN_CX, qubits[0], qubits[1], -1, 0.0))
+        elif gtype == GateType.cz:
+            ops.append((nat.QS_QNN_CZ, qubits[0], qubits[1], -1, 0.0))
+        else:
+            raise qerr.GateAlgorithmNotImplementError(
+                f"{gtype.name}: QNN programs take Rx, Ry, Rz, CX and CZ gates")
+    return ops
+
+
+class QNNProgramBatch:
+    """Circuits with their own gate programs (mixed ansaetze and widths), one
+    persistent launch for forward and adjoint backward (qs_qnn_batch_programs).
+
+    ``circuits``: ParameterizedCircuit per batch entry (parameters index the
+    shared theta); ``encode``: per circuit, (qubit, feature) pairs encoded as
+    Ry(x[c, feature]) before the circuit; loss = mean_c (<Z_0>_c - t_c)^2."""
+
+    def __init__(self, circuits, x, targets, nparams: int, encode=None, precision: str = "double",
+                 device: int = 0, config: Optional[Dict] = None):
+        self.ncirc = len(circuits)
+        self.x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.ncirc, -1)
+        self.nfeat = self.x.shape[1]
+        self.targets = np.ascontiguousarray(targets, dtype=np.float64)
+        if len(self.targets) != self.ncirc:
+            raise qerr.ValueError("circuits, x and targets must agree on the batch size")
+        self.nparams = int(nparams)
+        encode = encode or [()] * self.ncirc
+        recs, off = [], [0]
+        for pc, enc in zip(circuits, encode):
+            recs.extend(program_ops(pc, enc))
+            off.append(len(recs))
+        self.widths = np.array([pc.n for pc in circuits], dtype=np.int32)
+        self.offsets = np.array(off, dtype=np.int32)
+        self.ops = (nat.QsQnnOp * max(len(recs), 1))()
+        for i, r in enumerate(recs):
+            self.ops[i] = nat.QsQnnOp(*r)
+        self.precision = precision
+        self.dcode = nat.QS_C128 if precision == "double" else nat.QS_C64
+        self.device = device
+        self.lib = nat.load_library()
+        self.ctx = nat.context(device)
+        self.config = dict(config or {"lds_max_width": 11, "block_threads": 256, "blocks_per_cu": 2})
+
+    def forward_backward(self, theta: np.ndarray) -> Tuple[float, np.ndarray, np.ndarray, np.ndarray]:
+        theta = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if theta.size != self.nparams:
+            raise qerr.ValueError(f"theta needs {self.nparams} values")
+        ev = np.zeros(self.ncirc)
+        loss = ctypes.c_double(0)
+        gth = np.zeros(max(self.nparams, 1))
+        gx = np.zeros((self.ncirc, max(self.nfeat, 1)))
+        c = nat.QsQnnConfig()
+        c.lds_max_width = self.config["lds_max_width"]
+        c.block_threads = self.config["block_threads"]
+        c.blocks_per_cu = self.config["blocks_per_cu"]
+        P = ctypes.c_void_p
+        nat.check(self.lib.qs_qnn_batch_programs(
+            self.ctx, self.dcode, self.ncirc, self.widths.ctypes.data_as(P), self.offsets.ctypes.data_as(P),
+            ctypes.cast(self.ops, P), self.nparams, self.nfeat, self.x.ctypes.data_as(P),
+            self.targets.ctypes.data_as(P), theta.ctypes.data_as(P), ev.ctypes.data_as(P), ctypes.byref(loss),
+            gth.ctypes.data_as(P), gx.ctypes.data_as(P), ctypes.byref(c), nat.stream_ptr(self.device)),
+            "qs_qnn_batch_programs")
+        return loss.value, ev, gth[: self.nparams], gx[:, : self.nfeat]
