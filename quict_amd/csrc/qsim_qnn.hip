@@ -422,6 +422,7 @@ __global__ void k_qnn_reduce(const double* __restrict__ grad_circ, int ncirc, in
 struct QnnPlan {
   int dtype = -1, ncirc = 0, nfeat = 0, layers = 0, max_width = 0, lds_max_width = 0, threads = 0, blocks = 0,
       fuse = 0;
+  bool slots = false;  // the buffer holds the global state slots (some circuit wider than the LDS limit)
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t stream = nullptr;
@@ -452,7 +453,10 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   const int blocks = std::max(1, cus * per_cu);
   // device buffers: order, widths, x, target, theta, expval, loss, grad_theta, grad_x, counter,
   // per-circuit gradient terms, slots
-  const uint64_t slot_bytes = (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>);
+  // global state slots only when some circuit is wider than the LDS limit
+  int wmax = 0;
+  for (int i = 0; i < ncirc; ++i) wmax = std::max(wmax, (int)widths[i]);
+  const uint64_t slot_bytes = wmax > lds_w ? (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>) : 0;
   auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
   const uint64_t nth = (uint64_t)layers * 2 * max_width;
   const uint64_t sz_i = al(ncirc * sizeof(int32_t)), sz_x = al((uint64_t)ncirc * nfeat * sizeof(double)),
@@ -462,7 +466,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   QnnPlan& P = g_plan;
   const bool same = P.exec && P.dtype == dtype && P.ncirc == ncirc && P.nfeat == nfeat && P.layers == layers &&
                     P.max_width == max_width && P.lds_max_width == lds_w && P.threads == threads &&
-                    P.blocks == blocks && P.fuse == fuse;
+                    P.blocks == blocks && P.fuse == fuse && P.slots == (slot_bytes != 0);
   if (!same) {
     if (P.exec) (void)hipGraphExecDestroy(P.exec);
     if (P.graph) (void)hipGraphDestroy(P.graph);
@@ -558,6 +562,7 @@ static int qnn_t(qs_ctx* ctx, int dtype, int ncirc, const int32_t* widths, int n
   P.threads = threads;
   P.blocks = blocks;
   P.fuse = fuse;
+  P.slots = slot_bytes != 0;
   std::vector<double> lt(ncirc);
   QS_CHECK_HIP(hipMemcpyAsync(expval, d_e, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
   QS_CHECK_HIP(hipMemcpyAsync(lt.data(), d_l, ncirc * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -790,7 +795,7 @@ static int qnn_prog_t(qs_ctx* ctx, int ncirc, const int32_t* widths, const int32
                  sz_ops = al((uint64_t)std::max(nops, 1) * sizeof(qs_qnn_op)),
                  sz_x = al((uint64_t)ncirc * nf * sizeof(double)), sz_c = al((uint64_t)ncirc * sizeof(double)),
                  sz_t = al(np * sizeof(double)), sz_gc = al((uint64_t)ncirc * np * sizeof(double));
-  const uint64_t slot_bytes = (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>);
+  const uint64_t slot_bytes = max_width > lds_w ? (uint64_t)blocks * 2 * (1ull << max_width) * sizeof(cx<R>) : 0;
   const uint64_t total = 2 * sz_i + sz_o + sz_ops + 2 * sz_x + 3 * sz_c + 2 * sz_t + 256 + sz_gc + slot_bytes;
   int rc = ensure_scratch(ctx, total);
   if (rc) return rc;

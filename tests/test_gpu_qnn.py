@@ -205,3 +205,22 @@ def test_fusion_depths_agree_on_config_batch(precision, tol):
         assert abs(outs[fuse][0] - outs[1][0]) < tol
         assert np.max(np.abs(outs[fuse][1] - outs[1][1])) < tol
         assert np.max(np.abs(outs[fuse][2] - outs[1][2])) < tol
+
+
+def test_plan_reuse_when_a_wider_circuit_needs_the_global_slots():
+    """Two batches of one shape (so the captured plan could be reused): the first
+    fits LDS (no global state slots are allocated), the second has a circuit
+    wider than the LDS limit; both against the CPU oracle."""
+    say("qnn plan reuse, slots")
+    from quict_amd.gradient.qnn import QNNBatch
+    rng = np.random.RandomState(21)
+    x = rng.randn(3, 5)
+    t = np.array([1.0, -1.0, 1.0])
+    layers, mw = 2, 8
+    theta = rng.uniform(-np.pi, np.pi, layers * 2 * mw)
+    cfg = {"lds_max_width": 6, "block_threads": 128, "blocks_per_cu": 1, "use_graph": 1, "fuse_qubits": 2}
+    for widths in ([4, 5, 6], [4, 8, 6]):
+        w = np.array(widths, dtype=np.int32)
+        loss, ev, gth, gx = QNNBatch(w, x, t, layers=layers, max_width=mw, config=cfg).forward_backward(theta)
+        rl, rev, rg = reference(w, x, t, theta, layers, mw)
+        assert np.max(np.abs(ev - rev)) < 1e-12 and np.max(np.abs(gth - rg)) < 1e-10
