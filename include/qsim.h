@@ -257,6 +257,16 @@ int qs_sv_inner(qs_ctx* ctx, const void* a, const void* b, int n, int dtype, dou
 int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_gate* gates, int ngates,
                         const qs_gate* dgates, const int32_t* param_slot, double* grads, int nslots, void* stream);
 
+/* The same gradient for H given as a Pauli sum, with the energy: lam is
+   computed here as H psi (lam's prior contents are ignored) and *energy =
+   <psi|H|psi> (host) comes back with the gradient.  The sweep is planned
+   before any work is enqueued, so the planning overlaps the forward passes
+   still running on the stream; one host wait at the end (no counterpart in
+   the reference, whose gradient code lives in the external QuICT_ml). */
+int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_pauli_term* terms,
+                              int nterms, const qs_gate* gates, int ngates, const qs_gate* dgates,
+                              const int32_t* param_slot, double* grads, int nslots, double* energy, void* stream);
+
 /* ---- batched variational circuits (no reference counterpart) ------------- */
 /* Launch shape of the batched QNN kernel (picked by the autotuner):
    widths <= lds_max_width keep psi and lam in LDS (2 x 2^w amplitudes),

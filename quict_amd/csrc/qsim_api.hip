@@ -289,6 +289,23 @@ static int inner_t(qs_ctx* ctx, const cx<R>* a, const cx<R>* b, uint64_t N, doub
   return QS_OK;
 }
 
+int inner_partial_blocks() { return IP_BLOCKS; }
+
+int inner_async(const void* a, const void* b, int n, int dtype, double* part, double* res, hipStream_t s) {
+  const uint64_t N = 1ull << n;
+  const uint64_t blocks = std::min<uint64_t>(IP_BLOCKS, (N + 255) / 256);
+  if (dtype == QS_C128)
+    hipLaunchKernelGGL(k_inner_partial<double>, dim3((unsigned)blocks), dim3(256), 0, s, (const cx<double>*)a,
+                       (const cx<double>*)b, N, part);
+  else
+    hipLaunchKernelGGL(k_inner_partial<float>, dim3((unsigned)blocks), dim3(256), 0, s, (const cx<float>*)a,
+                       (const cx<float>*)b, N, part);
+  QS_CHECK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_inner_final, dim3(1), dim3(256), 0, s, part, (int)blocks, res);
+  QS_CHECK_HIP(hipGetLastError());
+  return QS_OK;
+}
+
 int inner_product(qs_ctx* ctx, const void* a, const void* b, int n, int dtype, double* re, double* im,
                   hipStream_t s) {
   if (dtype == QS_C128) return inner_t<double>(ctx, (const cx<double>*)a, (const cx<double>*)b, 1ull << n, re, im, s);
@@ -298,6 +315,37 @@ int inner_product(qs_ctx* ctx, const void* a, const void* b, int n, int dtype, d
 }  // namespace qs
 
 using namespace qs;
+
+namespace qs {
+int pauli_sum_async(qs_ctx* ctx, const void* vec, void* out, int n, int dtype, const qs_pauli_term* terms,
+                    int nterms, hipStream_t s) {
+  (void)ctx;
+  const uint64_t N = 1ull << n;
+  const unsigned blocks = (unsigned)((N + 255) / 256);
+  if (nterms == 0) {
+    QS_CHECK_HIP(hipMemsetAsync(out, 0, N * amp_bytes(dtype), s));
+    return QS_OK;
+  }
+  for (int first = 0; first < nterms; first += 64) {
+    PauliTerms pt;
+    pt.n = std::min(64, nterms - first);
+    for (int k = 0; k < pt.n; ++k) {
+      pt.coef[k] = terms[first + k].coef;
+      pt.x[k] = terms[first + k].xmask;
+      pt.z[k] = terms[first + k].zmask;
+      QS_REQUIRE(((pt.x[k] | pt.z[k]) >> n) == 0, "Pauli mask outside the register");
+    }
+    if (dtype == QS_C128)
+      hipLaunchKernelGGL(k_pauli_sum<double>, dim3(blocks), dim3(256), 0, s, (const cx<double>*)vec,
+                         (cx<double>*)out, N, pt, first, first > 0);
+    else
+      hipLaunchKernelGGL(k_pauli_sum<float>, dim3(blocks), dim3(256), 0, s, (const cx<float>*)vec,
+                         (cx<float>*)out, N, pt, first, first > 0);
+    QS_CHECK_HIP(hipGetLastError());
+  }
+  return QS_OK;
+}
+}  // namespace qs
 
 extern "C" {
 
@@ -486,31 +534,9 @@ int qs_sv_apply_pauli_sum(qs_ctx* ctx, const void* vec, void* out, int n, int dt
   int rc = check_state_args(vec, n, dtype);
   if (rc) return rc;
   QS_REQUIRE(ctx && out && out != vec && nterms >= 0 && (nterms == 0 || terms), "bad arguments");
-  const uint64_t N = 1ull << n;
-  const unsigned blocks = (unsigned)((N + 255) / 256);
-  if (nterms == 0) {
-    QS_CHECK_HIP(hipMemsetAsync(out, 0, N * amp_bytes(dtype), as_stream(stream)));
-    return QS_OK;
-  }
-  for (int first = 0; first < nterms; first += 64) {
-    PauliTerms pt;
-    pt.n = std::min(64, nterms - first);
-    for (int k = 0; k < pt.n; ++k) {
-      pt.coef[k] = terms[first + k].coef;
-      pt.x[k] = terms[first + k].xmask;
-      pt.z[k] = terms[first + k].zmask;
-      QS_REQUIRE(((pt.x[k] | pt.z[k]) >> n) == 0, "Pauli mask outside the register");
-    }
-    if (dtype == QS_C128)
-      hipLaunchKernelGGL(k_pauli_sum<double>, dim3(blocks), dim3(256), 0, as_stream(stream), (const cx<double>*)vec,
-                         (cx<double>*)out, N, pt, first, first > 0);
-    else
-      hipLaunchKernelGGL(k_pauli_sum<float>, dim3(blocks), dim3(256), 0, as_stream(stream), (const cx<float>*)vec,
-                         (cx<float>*)out, N, pt, first, first > 0);
-    QS_CHECK_HIP(hipGetLastError());
-  }
-  return QS_OK;
+  return pauli_sum_async(ctx, vec, out, n, dtype, terms, nterms, as_stream(stream));
 }
+
 
 int qs_sv_inner(qs_ctx* ctx, const void* a, const void* b, int n, int dtype, double* re, double* im, void* stream) {
   int rc = check_state_args(a, n, dtype);

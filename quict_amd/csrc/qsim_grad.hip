@@ -460,9 +460,14 @@ static void record_dense(const qs_gate& g, std::vector<double>& M) {
   }
 }
 
+// terms != nullptr: lam is computed here as H psi (and energy = <psi|H psi>)
+// after the sweep is planned, so the planning overlaps the forward passes
+// still running on the stream, and the energy comes back with the gradient
+// (one host wait).
 template <typename R>
 static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* gates, int ngates,
-                     const qs_gate* dgates, const int32_t* slot, double* grads, int nslots, hipStream_t s) {
+                     const qs_gate* dgates, const int32_t* slot, double* grads, int nslots, hipStream_t s,
+                     const qs_pauli_term* terms = nullptr, int nterms = 0, double* energy = nullptr) {
   int rc = QS_OK;
   std::vector<int> nb;
   std::vector<int> step_slot;
@@ -674,7 +679,8 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   // descriptors after the partial rows, sums and counts
   const uint64_t rows = std::max<size_t>(nb.size(), 1);
   const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * (sizeof(double) + sizeof(int)) + 1024 +
-                        mg.size() * sizeof(MGate) + ts.size() * sizeof(TStep) + 512;
+                        mg.size() * sizeof(MGate) + ts.size() * sizeof(TStep) + 512 +
+                        (2 * (uint64_t)inner_partial_blocks() + 64) * sizeof(double);
   rc = ensure_scratch(ctx, need);
   if (rc) return rc;
   double* partial = (double*)ctx->scratch;
@@ -684,6 +690,15 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   TStep* d_ts = (TStep*)(((uintptr_t)(d_mg + mg.size()) + 255) & ~(uintptr_t)255);
   if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_mg, mg.data(), mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
   if (!ts.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_ts, ts.data(), ts.size() * sizeof(TStep), hipMemcpyHostToDevice, s));
+  double* ipart = (double*)(((uintptr_t)(d_ts + ts.size()) + 255) & ~(uintptr_t)255);
+  double* ires = ipart + 2 * inner_partial_blocks();
+  const int dt = sizeof(R) == 8 ? QS_C128 : QS_C64;
+  if (energy) {
+    rc = pauli_sum_async(ctx, psi, lam, n, dt, terms, nterms, s);
+    if (rc) return rc;
+    rc = inner_async(psi, lam, n, dt, ipart, ires, s);
+    if (rc) return rc;
+  }
   rc = prof_begin(ctx, PROF_GRAD, s);
   if (rc) return rc;
   for (const Item& it : items) {
@@ -773,7 +788,10 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   if (rc) return rc;
   std::vector<double> h(nb.size());
   if (!nb.empty()) QS_CHECK_HIP(hipMemcpyAsync(h.data(), sums, nb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  double e2[2] = {0.0, 0.0};
+  if (energy) QS_CHECK_HIP(hipMemcpyAsync(e2, ires, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   QS_CHECK_HIP(hipStreamSynchronize(s));
+  if (energy) *energy = e2[0];
   for (size_t k = 0; k < nb.size(); ++k) {
     const int sl = step_slot[k];
     if (sl < nslots) grads[sl] += 2.0 * h[k];
@@ -809,6 +827,33 @@ int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, con
                              nslots, as_stream(stream));
   return adjoint_t<float>(ctx, (cx<float>*)psi, (cx<float>*)lam, n, gates, ngates, dgates, param_slot, grads, nslots,
                           as_stream(stream));
+}
+
+int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_pauli_term* terms,
+                              int nterms, const qs_gate* gates, int ngates, const qs_gate* dgates,
+                              const int32_t* param_slot, double* grads, int nslots, double* energy, void* stream) {
+  int rc = check_state_args(psi, n, dtype);
+  if (rc) return rc;
+  QS_REQUIRE(ctx && lam && lam != psi && energy, "bad state buffers or energy pointer");
+  QS_REQUIRE(nterms >= 1 && terms, "the Pauli sum needs at least one term");
+  for (int k = 0; k < nterms; ++k) QS_REQUIRE(((terms[k].xmask | terms[k].zmask) >> n) == 0, "Pauli mask outside the register");
+  QS_REQUIRE(ngates >= 0 && (ngates == 0 || (gates && param_slot)), "bad gate list");
+  QS_REQUIRE(nslots >= 0 && (nslots == 0 || grads), "bad gradient buffer");
+  for (int j = 0; j < ngates; ++j) {
+    rc = validate_gate(&gates[j], n);
+    if (rc) return rc;
+    if (param_slot[j] >= 0) {
+      QS_REQUIRE(dgates != nullptr, "derivative records missing");
+      QS_REQUIRE(param_slot[j] < nslots, "parameter slot out of range");
+      rc = validate_gate(&dgates[j], n);
+      if (rc) return rc;
+    }
+  }
+  if (dtype == QS_C128)
+    return adjoint_t<double>(ctx, (cx<double>*)psi, (cx<double>*)lam, n, gates, ngates, dgates, param_slot, grads,
+                             nslots, as_stream(stream), terms, nterms, energy);
+  return adjoint_t<float>(ctx, (cx<float>*)psi, (cx<float>*)lam, n, gates, ngates, dgates, param_slot, grads, nslots,
+                          as_stream(stream), terms, nterms, energy);
 }
 
 }  // extern "C"

@@ -37,6 +37,14 @@ int launch_dense_mfma(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* g
 // called once per context, with the context's device current.
 int noalloc_scratch_init(qs_ctx* ctx);
 
+// out = H vec for a Pauli sum (qsim_api.hip); enqueued on s, no host wait.
+int pauli_sum_async(qs_ctx* ctx, const void* vec, void* out, int n, int dtype, const qs_pauli_term* terms,
+                    int nterms, hipStream_t s);
+// <a|b> into res[0..1] (device), using part (device, 2 * inner_partial_blocks()
+// doubles); enqueued on s, no host wait (qsim_api.hip).
+int inner_async(const void* a, const void* b, int n, int dtype, double* part, double* res, hipStream_t s);
+int inner_partial_blocks();
+
 inline size_t amp_bytes(int dtype) { return dtype == QS_C128 ? 16 : 8; }
 
 }  // namespace qs

@@ -314,15 +314,20 @@ def adjoint_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, p
     theta = np.asarray(theta, dtype=np.float64)
     eng = _engine(pc.n, precision, device)
     fwd, inv, der, slot, keep = _records(pc, theta)
-    eng.forward(fwd)
-    E = eng.expectation(H)
     grads = np.zeros(max(pc.nparams, 1), dtype=np.float64)
-    nat.check(eng.lib.qs_adjoint_gradient(eng.ctx, eng.ptr(eng.psi), eng.ptr(eng.lam), pc.n, eng.dcode,
-                                          nat.gate_array_ptr(inv), len(inv), nat.gate_array_ptr(der),
-                                          slot.ctypes.data_as(ctypes.c_void_p),
-                                          grads.ctypes.data_as(ctypes.c_void_p), len(grads), eng.stream()),
-              "qs_adjoint_gradient")
-    return E, grads[: pc.nparams]
+    if not H.terms:
+        return 0.0, grads[: pc.nparams]
+    eng.forward(fwd)  # enqueued; the sweep below is planned while it runs
+    recs = H.records()
+    E = ctypes.c_double(0)
+    nat.check(eng.lib.qs_adjoint_gradient_pauli(eng.ctx, eng.ptr(eng.psi), eng.ptr(eng.lam), pc.n, eng.dcode,
+                                                ctypes.cast(recs, ctypes.c_void_p), len(recs),
+                                                nat.gate_array_ptr(inv), len(inv), nat.gate_array_ptr(der),
+                                                slot.ctypes.data_as(ctypes.c_void_p),
+                                                grads.ctypes.data_as(ctypes.c_void_p), len(grads), ctypes.byref(E),
+                                                eng.stream()),
+              "qs_adjoint_gradient_pauli")
+    return E.value, grads[: pc.nparams]
 
 
 def parameter_shift_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, precision: str = "double",
