@@ -276,7 +276,9 @@ struct TStep {
   int32_t p0, p1;        // tile-local positions of target 0 (matrix MSB) and target 1
   int32_t gi;            // parameterised: accumulator slot in the pass, else -1
   uint32_t cmask, cval;  // tile-local control bits and the values they need
-  int32_t pad[2];
+  int32_t twice;         // one target: a second step on the same bit and controls follows
+                         // (G^dagger in g[8..15], dG in d[8..15]), one LDS round trip for both
+  int32_t gi2;           // its accumulator slot, else -1
   double g[32];          // G^dagger, (re, im), 2^nt x 2^nt
   double d[32];          // dG/dtheta
 };
@@ -335,6 +337,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
       const TStep& S = steps[P.s0 + k];
       double acc = 0.0;
       const bool grad = S.gi >= 0;
+      double acc2 = 0.0;
+      const bool twice = S.twice != 0, grad2 = twice && S.gi2 >= 0;
       if (S.nt == 1) {
         const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
         for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
@@ -342,17 +346,33 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
           if ((i0 & S.cmask) != S.cval) continue;
           const uint32_t j0 = swz(i0), j1 = swz(i1);
           const cx<R> a0 = sp[j0], a1 = sp[j1], b0 = sl[j0], b1 = sl[j1];
-          const cx<R> pa = cfma(mul(g00, a0), g01, a1), pb = cfma(mul(g10, a0), g11, a1);
-          sl[j0] = cfma(mul(g00, b0), g01, b1);
-          sl[j1] = cfma(mul(g10, b0), g11, b1);
-          sp[j0] = pa;
-          sp[j1] = pb;
+          cx<R> pa = cfma(mul(g00, a0), g01, a1), pb = cfma(mul(g10, a0), g11, a1);
+          cx<R> la = cfma(mul(g00, b0), g01, b1), lb = cfma(mul(g10, b0), g11, b1);
           if (grad) {
             const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), pa), cget<R>(S.d, 1), pb);
             const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), pa), cget<R>(S.d, 3), pb);
             acc += (double)b0.re * (double)m0.re + (double)b0.im * (double)m0.im + (double)b1.re * (double)m1.re +
                    (double)b1.im * (double)m1.im;
           }
+          if (twice) {  // the following step, on the same pair
+            const cx<R> h00 = cget<R>(S.g, 4), h01 = cget<R>(S.g, 5), h10 = cget<R>(S.g, 6), h11 = cget<R>(S.g, 7);
+            const cx<R> qa = cfma(mul(h00, pa), h01, pb), qb = cfma(mul(h10, pa), h11, pb);
+            if (grad2) {
+              const cx<R> m0 = cfma(mul(cget<R>(S.d, 4), qa), cget<R>(S.d, 5), qb);
+              const cx<R> m1 = cfma(mul(cget<R>(S.d, 6), qa), cget<R>(S.d, 7), qb);
+              acc2 += (double)la.re * (double)m0.re + (double)la.im * (double)m0.im + (double)lb.re * (double)m1.re +
+                      (double)lb.im * (double)m1.im;
+            }
+            const cx<R> ma = cfma(mul(h00, la), h01, lb), mb = cfma(mul(h10, la), h11, lb);
+            pa = qa;
+            pb = qb;
+            la = ma;
+            lb = mb;
+          }
+          sp[j0] = pa;
+          sp[j1] = pb;
+          sl[j0] = la;
+          sl[j1] = lb;
         }
       } else {
         const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
@@ -394,6 +414,11 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if ((tid & 63u) == 0) red[S.gi * NW + wave] += acc;  // one writer per (slot, wave)
+      }
+      if (grad2) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc2 += __shfl_down(acc2, o, 64);
+        if ((tid & 63u) == 0) red[S.gi2 * NW + wave] += acc2;
       }
       __syncthreads();
     }
@@ -513,7 +538,6 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
       if ((u >> b) & 1) P.ubits[nbits++] = b;
     P.nb = nbits;
     P.s0 = (int)ts.size();
-    P.ns = (int)cur.size();
     const uint64_t ntiles = 1ull << (n - nbits);
     const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
     auto local = [&](int bit) {
@@ -529,16 +553,33 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
       S.p0 = local(g.bits[g.ncontrols]);
       S.p1 = g.ntargets > 1 ? local(g.bits[g.ncontrols + 1]) : 0;
       S.gi = -1;
+      S.gi2 = -1;
       for (int i = 0; i < g.ncontrols; ++i) {
         S.cmask |= 1u << local(g.bits[i]);
         S.cval |= 1u << local(g.bits[i]);
       }
-      std::vector<double> M;
+      std::vector<double> M, Dm;
       record_dense(g, M);
+      if (slot[j] >= 0) record_dense(dgates[j], Dm);
+      // a one-target step on the bit and controls of the previous one-target
+      // step of the pass joins it (same pairs: one LDS round trip)
+      TStep* prev = (int)ts.size() > P.s0 ? &ts.back() : nullptr;
+      if (S.nt == 1 && prev && prev->nt == 1 && !prev->twice && prev->p0 == S.p0 && prev->cmask == S.cmask &&
+          prev->cval == S.cval) {
+        prev->twice = 1;
+        for (size_t i = 0; i < M.size(); ++i) prev->g[8 + i] = M[i];
+        if (slot[j] >= 0) {
+          for (size_t i = 0; i < Dm.size(); ++i) prev->d[8 + i] = Dm[i];
+          prev->gi2 = P.ngrad;
+          P.row[P.ngrad++] = (int)nb.size();
+          nb.push_back(blocks);
+          step_slot.push_back(slot[j]);
+        }
+        continue;
+      }
       for (size_t i = 0; i < M.size(); ++i) S.g[i] = M[i];
       if (slot[j] >= 0) {
-        record_dense(dgates[j], M);
-        for (size_t i = 0; i < M.size(); ++i) S.d[i] = M[i];
+        for (size_t i = 0; i < Dm.size(); ++i) S.d[i] = Dm[i];
         S.gi = P.ngrad;
         P.row[P.ngrad++] = (int)nb.size();
         nb.push_back(blocks);
@@ -546,6 +587,7 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
       }
       ts.push_back(S);
     }
+    P.ns = (int)ts.size() - P.s0;
     items.push_back({2, (int)tpasses.size()});
     tpasses.push_back(P);
     cur.clear();
