@@ -648,47 +648,71 @@ template <typename R> static void pack_program(const FusedProgram& prog, ProgIma
   for (uint64_t i = 0; i < ncoef; ++i) hc[i] = mk<R>((R)prog.coef[2 * i], (R)prog.coef[2 * i + 1]);
 }
 
-// Launch every pass of `prog` (image at dprog) on stream s.  prof: bracket the
-// launches with the context's timing events (never while capturing a graph).
+// Device view of an uploaded image.
+template <typename R> struct ProgDev {
+  const PassHdr* H;
+  const FOpW* O;
+  const uint32_t* X;
+  const cx<R>* C;
+  ProgDev(const char* d, const ProgImage& im)
+      : H((const PassHdr*)d), O((const FOpW*)(d + im.hb)), X((const uint32_t*)(d + im.hb + im.ob)),
+        C((const cx<R>*)(d + im.hb + im.ob + im.xb)) {}
+};
+
+// One fused pass: host header hd, its device copy dh.  prof: bracket the
+// launch with the context's timing events (never while capturing a graph).
+template <typename R, int RB, int TB>
+static int launch_fused(qs_ctx* ctx, cx<R>* v, int n, const PassHdr& hd, const PassHdr* dh, const ProgDev<R>& d,
+                        hipStream_t s, bool prof) {
+  constexpr int K = RB + TB;
+  const uint64_t tiles = 1ull << (n - K);
+  const size_t lds_bytes = (size_t)(1u << (K - 1)) * sizeof(cx<R>);  // half-tile relayout image
+  const size_t lds_relayout = hd.uses_lds ? lds_bytes : 0;
+  const size_t lds = hd.has_dk ? lds_relayout : lds_relayout + (size_t)hd.ncoef * sizeof(cx<R>);
+  int rc = QS_OK;
+  if (prof && (rc = prof_begin(ctx, PROF_FUSED, s))) return rc;
+  if (hd.has_dk) {
+    hipLaunchKernelGGL((k_fused<R, RB, TB, true, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dh, d.O,
+                       d.C, d.X, tiles);
+  } else if (ctx->fused_prefetch) {
+    // persistent, prefetching: resident workgroups only
+    const uint64_t grid = std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * ctx->fused_wg_per_cu);
+    hipLaunchKernelGGL((k_fused<R, RB, TB, false, true>), dim3((unsigned)grid), dim3(1 << TB), lds, s, v, dh, d.O,
+                       d.C, d.X, tiles);
+  } else {
+    hipLaunchKernelGGL((k_fused<R, RB, TB, false, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dh, d.O,
+                       d.C, d.X, tiles);
+  }
+  QS_CHECK_HIP(hipGetLastError());
+  if (prof && (rc = prof_end(ctx, s))) return rc;
+  return QS_OK;
+}
+
+// A pass the tile kernel cannot take: the gate's own kernel.
+template <typename R>
+static int launch_single(qs_ctx* ctx, cx<R>* v, int n, const qs_gate& g, hipStream_t s, bool prof) {
+  int rc = QS_OK;
+  if (prof && (rc = prof_begin(ctx, PROF_GATE, s))) return rc;
+  rc = apply_gate(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &g, s);
+  if (rc) return rc;
+  if (prof && (rc = prof_end(ctx, s))) return rc;
+  return QS_OK;
+}
+
+// Launch every pass of `prog` (image at dprog) on stream s.
 template <typename R, int RB, int TB>
 static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, const FusedProgram& prog,
                          const char* dprog, const ProgImage& im, hipStream_t s, bool prof, int* npass_out) {
-  constexpr int K = RB + TB;
-  const PassHdr* dH = (const PassHdr*)dprog;
-  const FOpW* dO = (const FOpW*)(dprog + im.hb);
-  const uint32_t* dX = (const uint32_t*)(dprog + im.hb + im.ob);
-  const cx<R>* dC = (const cx<R>*)(dprog + im.hb + im.ob + im.xb);
-  const uint64_t tiles = 1ull << (n - K);
-  const size_t lds_bytes = (size_t)(1u << (K - 1)) * sizeof(cx<R>);  // half-tile relayout image
+  const ProgDev<R> d(dprog, im);
   int npass = 0, rc = QS_OK;
   for (size_t i = 0; i < prog.plans.size(); ++i) {
     if (!prog.plans[i].fused) {
-      if (prof && (rc = prof_begin(ctx, PROF_GATE, s))) return rc;
-      rc = apply_gate(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, &prog.pgates[prog.plans[i].single], s);
-      if (rc) return rc;
-      if (prof && (rc = prof_end(ctx, s))) return rc;
-      ++npass;
-      continue;
-    }
-    const int h = prog.hdr_of[i];
-    const PassHdr& hd = prog.hdrs[h];
-    const size_t lds_relayout = hd.uses_lds ? lds_bytes : 0;
-    const size_t lds = hd.has_dk ? lds_relayout : lds_relayout + (size_t)hd.ncoef * sizeof(cx<R>);
-    if (prof && (rc = prof_begin(ctx, PROF_FUSED, s))) return rc;
-    if (hd.has_dk) {
-      hipLaunchKernelGGL((k_fused<R, RB, TB, true, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h,
-                         dO, dC, dX, tiles);
-    } else if (ctx->fused_prefetch) {
-      // persistent, prefetching: resident workgroups only
-      const uint64_t grid = std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * ctx->fused_wg_per_cu);
-      hipLaunchKernelGGL((k_fused<R, RB, TB, false, true>), dim3((unsigned)grid), dim3(1 << TB), lds, s, v, dH + h,
-                         dO, dC, dX, tiles);
+      rc = launch_single<R>(ctx, v, n, prog.pgates[prog.plans[i].single], s, prof);
     } else {
-      hipLaunchKernelGGL((k_fused<R, RB, TB, false, false>), dim3((unsigned)tiles), dim3(1 << TB), lds, s, v, dH + h,
-                         dO, dC, dX, tiles);
+      const int h = prog.hdr_of[i];
+      rc = launch_fused<R, RB, TB>(ctx, v, n, prog.hdrs[h], d.H + h, d, s, prof);
     }
-    QS_CHECK_HIP(hipGetLastError());
-    if (prof && (rc = prof_end(ctx, s))) return rc;
+    if (rc) return rc;
     ++npass;
   }
   if (npass_out) *npass_out = npass;
@@ -706,10 +730,52 @@ static Config fused_config(int RB, int TB, int L, bool moves) {
   return c;
 }
 
+// Planning overlapped with the passes, for plans without qubit moves: each
+// chunk of passes is uploaded and launched as soon as it is planned, so the
+// device starts after the first pass is planned instead of after the whole
+// program.  A planner error after the first chunk leaves the vector partly
+// evolved (the planner's errors are internal: a gate the tile kernel takes
+// always schedules).
+template <typename R, int RB, int TB>
+static int run_fused_streamed(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, const Config& c,
+                              int* passes, hipStream_t s) {
+  FusedProgram prog;
+  int npass = 0;
+  auto chunk = [&](const FusedProgram& sub, int) -> int {
+    ProgImage im;
+    pack_program<R>(sub, im);
+    qs_ctx::ProgSlot* slot = nullptr;
+    int rc = acquire_prog_slot(ctx, im.host.size() + 256, &slot);
+    if (rc) return rc;
+    std::memcpy(slot->host, im.host.data(), im.host.size());
+    QS_CHECK_HIP(hipMemcpyAsync(slot->dev, slot->host, im.host.size(), hipMemcpyHostToDevice, s));
+    const ProgDev<R> d((const char*)slot->dev, im);
+    for (size_t h = 0; h < sub.hdrs.size() && !rc; ++h, ++npass)
+      rc = launch_fused<R, RB, TB>(ctx, v, n, sub.hdrs[h], d.H + h, d, s, true);
+    QS_CHECK_HIP(hipEventRecord(slot->done, s));
+    slot->pending = true;
+    return rc;
+  };
+  auto single = [&](const qs_gate& g) -> int {
+    ++npass;
+    return launch_single<R>(ctx, v, n, g, s, true);
+  };
+  int rc = build_streamed(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog, chunk, single);
+  if (!rc)
+    if (const char* dbg = getenv("QS_FUSION_DEBUG"))
+      if (dbg[0] == '1') print_program_stats(prog, n, RB + TB, c.L, ngates);
+  if (passes) *passes = npass;
+  return rc;
+}
+
 template <typename R, int RB, int TB>
 static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, bool moves,
                        int* passes, hipStream_t s) {
   const Config c = fused_config(RB, TB, L, moves);
+  if (c.remap == 0) {
+    const char* st = getenv("QS_FUSION_STREAM");
+    if (!(st && st[0] == '0')) return run_fused_streamed<R, RB, TB>(ctx, v, n, gates, ngates, c, passes, s);
+  }
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;

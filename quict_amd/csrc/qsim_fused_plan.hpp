@@ -718,6 +718,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
     for (int i = 0; i < g.ncontrols + g.ntargets; ++i) g.bits[i] = phys[gates[gi].bits[i]];
   };
   std::vector<char> taken(ngates, 0);
+  const uint64_t all_bits = n >= 64 ? ~0ull : (1ull << n) - 1;
   while (!remaining.empty()) {
     PassPlan p;
     p.fused = true;
@@ -752,6 +753,7 @@ static void plan_groups(const qs_gate* gates, int ngates, int n, const Config& c
         }
         wait_all |= m;
         wait_mix |= gx[gi];
+        if (wait_mix == all_bits) break;  // every later gate shares a bit some waiting gate mixes
       }
       if (best < 0) break;
       taken[best] = 1;
@@ -938,6 +940,7 @@ struct FusedProgram {
   std::vector<FOp> ops;
   std::vector<double> coef;  // (re, im)
   std::vector<uint32_t> aux;
+  std::vector<int> aux0;     // header -> first aux word of its relayouts
 };
 
 
@@ -1330,16 +1333,22 @@ static int schedule_pass(const qs_gate* gates, PassPlan& p, const Config& c, uin
 }
 
 // Plan and schedule one program (with `remap`: moving qubits onto the low bits).
+// `ready`, when given, sees each fused pass as soon as it is scheduled (its
+// header is the last one; the plans before it are final, the pass itself is
+// appended to prog.plans after the call); a nonzero return stops the build.
 static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, bool remap, uint32_t amp_bytes,
-                     FusedProgram& prog) {
+                     FusedProgram& prog, const std::function<int(PassPlan&)>& ready = {}) {
   int rc = QS_OK;
   auto on_pass = [&](PassPlan& p) {
     if (rc) return;
     PassHdr H;
+    const int a0 = (int)prog.aux.size();
     rc = schedule_pass(prog.pgates.data(), p, c, amp_bytes, H, prog);
     if (rc) return;
     p.hdr = (int)prog.hdrs.size();
     prog.hdrs.push_back(H);
+    prog.aux0.push_back(a0);
+    if (ready) rc = ready(p);
   };
   plan_groups(gates, ngates, n, c, remap, prog.plans, prog.pgates, on_pass);
   if (rc) return rc;
@@ -1351,8 +1360,9 @@ static int build_one(const qs_gate* gates, int ngates, int n, const Config& c, b
 
 // The tile kernel's op switch treats every encoding outside these ranges as
 // unreachable: a program is checked here before it can reach a launch.
-static int check_program(const FusedProgram& prog, const Config& c) {
-  for (const PassHdr& h : prog.hdrs) {
+static int check_passes(const FusedProgram& prog, const Config& c, int h0, int h1) {
+  for (int hi = h0; hi < h1; ++hi) {
+    const PassHdr& h = prog.hdrs[hi];
     for (int i = 0; i < h.nops; ++i) {
       const FOp& o = prog.ops[(size_t)h.op0 + i];
       bool ok = o.kind >= F_RELAYOUT && o.kind <= F_XX2;
@@ -1386,14 +1396,24 @@ static int check_program(const FusedProgram& prog, const Config& c) {
   return QS_OK;
 }
 
-static int build_program_unchecked(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
-                                   FusedProgram& prog) {
+static int check_program(const FusedProgram& prog, const Config& c) {
+  return check_passes(prog, c, 0, (int)prog.hdrs.size());
+}
+
+static int check_config(const Config& c) {
   // every tile keeps its L low bits on thread slots at the load and the store
   if (c.L < 1 || c.L > c.TB) {
     set_error("fusion planner: low_bits must be in [1, tile_log2 - 4]");
     return QS_ERR_ARG;
   }
-  int rc = build_one(gates, ngates, n, c, false, amp_bytes, prog);
+  return QS_OK;
+}
+
+static int build_program_unchecked(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
+                                   FusedProgram& prog) {
+  int rc = check_config(c);
+  if (rc) return rc;
+  rc = build_one(gates, ngates, n, c, false, amp_bytes, prog);
   if (rc || c.remap == 0 || n < c.RB + c.TB + 2) return rc;
   // moving qubits onto the low bits pays when it saves enough passes (the
   // passes that put the qubits back included): the passes get heavier.  Short
@@ -1410,6 +1430,74 @@ static int build_program(const qs_gate* gates, int ngates, int n, const Config& 
                          FusedProgram& prog) {
   const int rc = build_program_unchecked(gates, ngates, n, c, amp_bytes, prog);
   return rc ? rc : check_program(prog, c);
+}
+
+// The passes hdrs[h0, h1) of a program (possibly still being built) as a
+// program of their own: headers rebased onto the chunk's ops, coefficients
+// and relayout constants (a relayout op's coef is its first aux word).
+static void chunk_program(const FusedProgram& prog, int h0, int h1, FusedProgram& sub) {
+  const PassHdr& a = prog.hdrs[h0];
+  const PassHdr& z = prog.hdrs[h1 - 1];
+  const int o0 = a.op0, o1 = z.op0 + z.nops, c0 = a.coef0, c1 = z.coef0 + z.ncoef;
+  const int x0 = prog.aux0[h0], x1 = h1 < (int)prog.aux0.size() ? prog.aux0[h1] : (int)prog.aux.size();
+  sub.hdrs.assign(prog.hdrs.begin() + h0, prog.hdrs.begin() + h1);
+  for (PassHdr& h : sub.hdrs) {
+    h.op0 -= o0;
+    h.coef0 -= c0;
+  }
+  sub.ops.assign(prog.ops.begin() + o0, prog.ops.begin() + o1);
+  for (FOp& o : sub.ops)
+    if (o.kind == F_RELAYOUT) o.coef -= x0;
+  sub.aux.assign(prog.aux.begin() + x0, prog.aux.begin() + x1);
+  sub.coef.assign(prog.coef.begin() + 2 * (size_t)c0, prog.coef.begin() + 2 * (size_t)c1);
+}
+
+// Plan without qubit moves (each pass is final once scheduled) and hand the
+// passes over, in plan order, as they are planned: runs of fused passes as
+// checked chunks of 1, 2, 4, ... 16 passes (`chunk(sub, first pass index)`, sub
+// from chunk_program), passes the tile kernel cannot take one at a time
+// (`single(gate record)`).  The device can start on the first chunk while the
+// rest is planned.  A nonzero return of a callback stops the build.
+static int build_streamed(const qs_gate* gates, int ngates, int n, const Config& c, uint32_t amp_bytes,
+                          FusedProgram& prog, const std::function<int(const FusedProgram&, int)>& chunk,
+                          const std::function<int(const qs_gate&)>& single) {
+  int rc = check_config(c);
+  if (rc) return rc;
+  size_t next_plan = 0;  // plans before this one are handed over or in the open chunk
+  int h0 = 0, h1 = 0;    // open chunk: headers [h0, h1)
+  int want = 1;
+  auto flush = [&]() -> int {
+    if (h1 == h0) return QS_OK;
+    int r = check_passes(prog, c, h0, h1);
+    if (r) return r;
+    FusedProgram sub;
+    chunk_program(prog, h0, h1, sub);
+    r = chunk(sub, h0);
+    h0 = h1;
+    want = std::min(2 * want, 16);
+    return r;
+  };
+  auto singles = [&](size_t upto) -> int {  // single-gate passes planned since the last fused one
+    for (; next_plan < upto; ++next_plan) {
+      const PassPlan& q = prog.plans[next_plan];
+      if (q.fused) continue;
+      int r = flush();
+      if (!r) r = single(prog.pgates[q.single]);
+      if (r) return r;
+    }
+    return QS_OK;
+  };
+  auto ready = [&](PassPlan& p) -> int {
+    const int r = singles(prog.plans.size());
+    if (r) return r;
+    next_plan = prog.plans.size() + 1;  // p itself is appended next
+    h1 = p.hdr + 1;
+    return h1 - h0 >= want ? flush() : QS_OK;
+  };
+  rc = build_one(gates, ngates, n, c, false, amp_bytes, prog, ready);
+  if (!rc) rc = singles(prog.plans.size());
+  if (!rc) rc = flush();
+  return rc;
 }
 
 static void print_program_stats(const FusedProgram& prog, int n, int K, int L, int ngates) {

@@ -646,17 +646,19 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   // commute, so the gradient of the same function comes out), fewest new tile
   // bits first; the pass applies its steps in backward order.
   std::vector<int> run;
+  const uint64_t all_bits = n >= 64 ? ~0ull : (1ull << n) - 1;
   auto plan_run = [&]() {
     std::vector<char> done(run.size(), 0);
-    size_t left = run.size();
+    size_t left = run.size(), first = 0;  // steps before `first` are all placed
     while (left) {
       uint64_t S = lowmask;
       int ng = 0, np = 0;
       std::vector<char> pick(run.size(), 0);
+      while (first < run.size() && done[first]) ++first;
       while (np < AT_MAX_STEPS) {
         uint64_t wait = 0;
         int best = -1, best_new = 1 << 30;
-        for (size_t k = 0; k < run.size(); ++k) {
+        for (size_t k = first; k < run.size() && wait != all_bits; ++k) {  // all bits waiting: nothing may join
           if (done[k] || pick[k]) continue;
           const int g = run[k];
           const uint64_t m = gbits(g);
@@ -718,22 +720,42 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   }
   if (tile) plan_run();
   flush();
-  // descriptors after the partial rows, sums and counts
+  // scratch: partial rows, sums, the energy's partials; descriptors (pass
+  // steps, block counts) go up in one copy from a pinned slot (a copy from
+  // pageable memory would wait for the forward passes still queued), which
+  // also takes the results back
   const uint64_t rows = std::max<size_t>(nb.size(), 1);
-  const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * (sizeof(double) + sizeof(int)) + 1024 +
-                        mg.size() * sizeof(MGate) + ts.size() * sizeof(TStep) + 512 +
+  const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * sizeof(double) + 512 +
                         (2 * (uint64_t)inner_partial_blocks() + 64) * sizeof(double);
   rc = ensure_scratch(ctx, need);
   if (rc) return rc;
   double* partial = (double*)ctx->scratch;
   double* sums = partial + rows * ADJ_MAX_BLOCKS;
-  int* d_nb = (int*)(sums + rows);
-  MGate* d_mg = (MGate*)(((uintptr_t)(d_nb + rows) + 255) & ~(uintptr_t)255);
-  TStep* d_ts = (TStep*)(((uintptr_t)(d_mg + mg.size()) + 255) & ~(uintptr_t)255);
-  if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_mg, mg.data(), mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
-  if (!ts.empty()) QS_CHECK_HIP(hipMemcpyAsync(d_ts, ts.data(), ts.size() * sizeof(TStep), hipMemcpyHostToDevice, s));
-  double* ipart = (double*)(((uintptr_t)(d_ts + ts.size()) + 255) & ~(uintptr_t)255);
+  double* ipart = (double*)(((uintptr_t)(sums + rows) + 255) & ~(uintptr_t)255);
   double* ires = ipart + 2 * inner_partial_blocks();
+  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+  const uint64_t ts_off = al(mg.size() * sizeof(MGate)), nb_off = al(ts_off + ts.size() * sizeof(TStep));
+  const uint64_t up = nb_off + rows * sizeof(int), res_off = al(up);
+  qs_ctx::ProgSlot* pslot = nullptr;
+  rc = acquire_prog_slot(ctx, res_off + (rows + 2) * sizeof(double), &pslot);
+  if (rc) return rc;
+  struct SlotGuard {  // the slot is busy until everything enqueued below has run
+    qs_ctx::ProgSlot* sl;
+    hipStream_t st;
+    ~SlotGuard() {
+      (void)hipEventRecord(sl->done, st);
+      sl->pending = true;
+    }
+  } guard{pslot, s};
+  char* hs = pslot->host;
+  if (!mg.empty()) std::memcpy(hs, mg.data(), mg.size() * sizeof(MGate));
+  if (!ts.empty()) std::memcpy(hs + ts_off, ts.data(), ts.size() * sizeof(TStep));
+  if (!nb.empty()) std::memcpy(hs + nb_off, nb.data(), nb.size() * sizeof(int));
+  QS_CHECK_HIP(hipMemcpyAsync(pslot->dev, hs, up, hipMemcpyHostToDevice, s));
+  const MGate* d_mg = (const MGate*)pslot->dev;
+  const TStep* d_ts = (const TStep*)((char*)pslot->dev + ts_off);
+  const int* d_nb = (const int*)((char*)pslot->dev + nb_off);
+  double* h_res = (double*)(hs + res_off);  // grads rows, then the energy (re, im)
   const int dt = sizeof(R) == 8 ? QS_C128 : QS_C64;
   if (energy) {
     rc = pauli_sum_async(ctx, psi, lam, n, dt, terms, nterms, s);
@@ -821,22 +843,20 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     QS_CHECK_HIP(hipGetLastError());
   }
   if (!nb.empty()) {
-    QS_CHECK_HIP(hipMemcpyAsync(d_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_adj_reduce, dim3((unsigned)nb.size()), dim3(256), 0, s, partial, d_nb, ADJ_MAX_BLOCKS,
                        (int)nb.size(), sums);
     QS_CHECK_HIP(hipGetLastError());
   }
   rc = prof_end(ctx, s);
   if (rc) return rc;
-  std::vector<double> h(nb.size());
-  if (!nb.empty()) QS_CHECK_HIP(hipMemcpyAsync(h.data(), sums, nb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  double e2[2] = {0.0, 0.0};
-  if (energy) QS_CHECK_HIP(hipMemcpyAsync(e2, ires, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (!nb.empty())
+    QS_CHECK_HIP(hipMemcpyAsync(h_res, sums, nb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (energy) QS_CHECK_HIP(hipMemcpyAsync(h_res + rows, ires, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   QS_CHECK_HIP(hipStreamSynchronize(s));
-  if (energy) *energy = e2[0];
+  if (energy) *energy = h_res[rows];
   for (size_t k = 0; k < nb.size(); ++k) {
     const int sl = step_slot[k];
-    if (sl < nslots) grads[sl] += 2.0 * h[k];
+    if (sl < nslots) grads[sl] += 2.0 * h_res[k];
   }
   return QS_OK;
 }

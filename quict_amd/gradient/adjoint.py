@@ -50,8 +50,11 @@ class PauliSum:
         for _, s in self.terms:
             if len(s) != n or any(ch not in "IXYZ" for ch in s):
                 raise qerr.ValueError("Pauli strings must have one I/X/Y/Z letter per qubit")
+        self._recs = None  # (terms they were built from, qs_pauli_term records)
 
     def records(self) -> np.ndarray:
+        if self._recs is not None and self._recs[0] == self.terms:
+            return self._recs[1]
         arr = (nat.QsPauliTerm * len(self.terms))()
         for k, (c, s) in enumerate(self.terms):
             x = z = 0
@@ -62,6 +65,7 @@ class PauliSum:
                 if ch in "ZY":
                     z |= bit
             arr[k].coef, arr[k].xmask, arr[k].zmask = c, x, z
+        self._recs = (list(self.terms), arr)
         return arr
 
     def matrix(self) -> np.ndarray:
@@ -88,6 +92,7 @@ class ParameterizedCircuit:
         self.ops: List[Tuple[GateType, List[int], Optional[int], Optional[list]]] = []
         self.nparams = 0
         self._template = None   # record templates, rebuilt after every add()
+        self._work = None       # working copies of the templates' records
 
     def add(self, gtype: GateType, qubits, param: Optional[int] = None, value=None):
         qubits = [qubits] if isinstance(qubits, int) else list(qubits)
@@ -203,19 +208,24 @@ def _template(pc: ParameterizedCircuit):
     return pc._template
 
 
-def _records(pc: ParameterizedCircuit, theta: np.ndarray):
+def _records(pc: ParameterizedCircuit, theta: np.ndarray, reuse: bool = False):
     """Forward, inverse and derivative records of pc at theta: the structure's
     template with the rotation payloads refilled in closed form (vectorised);
-    other parameterised gates go through the generic compiler."""
+    other parameterised gates go through the generic compiler.  reuse: refill
+    the circuit's own working copies instead of new arrays (valid until the
+    next such call; for callers that hand them to one native call)."""
     fwd0, inv0, der0, slot, keep0, fast, generic = _template(pc)
-    fwd, inv, der = fwd0.copy(), inv0.copy(), der0.copy()
+    if reuse:
+        if pc._work is None or pc._work[0] is not fwd0:
+            pc._work = (fwd0, fwd0.copy(), inv0.copy(), der0.copy())
+        fwd, inv, der = pc._work[1:]
+    else:
+        fwd, inv, der = fwd0.copy(), inv0.copy(), der0.copy()
     keep = list(keep0)
     for gt, (rows, params) in fast.items():
         f, i, d = _rotation_payloads(gt, np.asarray(theta, dtype=np.float64)[params])
         for arr, pay in ((fwd, f), (inv, i), (der, d)):
-            m = arr["m"][rows]
-            m[:, : 2 * pay.shape[1]] = pay.view(np.float64)
-            arr["m"][rows] = m
+            arr["m"][rows, : 2 * pay.shape[1]] = pay.view(np.float64)
     if generic:
         for j in generic:
             g = pc.gate(j, theta)
@@ -313,7 +323,7 @@ def adjoint_gradient(pc: ParameterizedCircuit, theta: np.ndarray, H: PauliSum, p
                      device: int = 0) -> Tuple[float, np.ndarray]:
     theta = np.asarray(theta, dtype=np.float64)
     eng = _engine(pc.n, precision, device)
-    fwd, inv, der, slot, keep = _records(pc, theta)
+    fwd, inv, der, slot, keep = _records(pc, theta, reuse=True)
     grads = np.zeros(max(pc.nparams, 1), dtype=np.float64)
     if not H.terms:
         return 0.0, grads[: pc.nparams]

@@ -246,6 +246,8 @@ extern "C" const char* emu_last_error() { return qs::g_err.c_str(); }
 // store moves qubits between index bits.  QS_EMU_FUSE=2 plans as
 // qs_sv_apply_gates with fuse=2 (qubit moves when they save at least 8% of the
 // passes); QS_FUSION_REMAP=1 moves qubits whenever that saves a pass.
+// QS_EMU_STREAM=1 (plans without moves) runs the passes chunk by chunk as
+// the fuse=1 path launches them.
 extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, int RB, int TB, int L, int amp_bytes,
                          int* stats) {
   using namespace qs;
@@ -255,23 +257,44 @@ extern "C" int emu_apply(double* vec, int n, const qs_gate* gates, int ngates, i
     if (const char* fu = getenv("QS_EMU_FUSE")) c.remap = atoi(fu) == 2 ? 1 : 0;  // qs_sv_apply_gates' fuse
     if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = atoi(rm) == 1 ? 2 : 0;
     FusedProgram prog;
-    int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);
-    if (rc) return rc;
     zc* v = reinterpret_cast<zc*>(vec);
     int maxc = 0;
-    for (size_t i = 0; i < prog.plans.size(); ++i) {
-      if (!vec) {
-        if (prog.plans[i].fused) maxc = std::max(maxc, prog.hdrs[prog.hdr_of[i]].ncoef);
-        continue;
+    const char* stream = getenv("QS_EMU_STREAM");
+    if (stream && stream[0] == '1' && c.remap == 0) {
+      // as qs_sv_apply_gates(fuse=1) runs it: passes run chunk by chunk, from
+      // the rebased chunk programs, while the rest is still being planned
+      auto chunk = [&](const FusedProgram& sub, int) -> int {
+        for (const PassHdr& H : sub.hdrs) {
+          maxc = std::max(maxc, H.ncoef);
+          if (vec)
+            emu_pass(v, n, RB, TB, (uint32_t)amp_bytes, H, sub.ops.data() + H.op0, sub.coef.data() + 2 * H.coef0,
+                     sub.aux.data());
+        }
+        return 0;
+      };
+      auto single = [&](const qs_gate& g) -> int {
+        if (vec) cpu_apply_gates(vec, n, &g, 1, 1);
+        return 0;
+      };
+      int rc = build_streamed(gates, ngates, n, c, (uint32_t)amp_bytes, prog, chunk, single);
+      if (rc) return rc;
+    } else {
+      int rc = build_program(gates, ngates, n, c, (uint32_t)amp_bytes, prog);
+      if (rc) return rc;
+      for (size_t i = 0; i < prog.plans.size(); ++i) {
+        if (!vec) {
+          if (prog.plans[i].fused) maxc = std::max(maxc, prog.hdrs[prog.hdr_of[i]].ncoef);
+          continue;
+        }
+        if (!prog.plans[i].fused) {
+          cpu_apply_gates(vec, n, &prog.pgates[prog.plans[i].single], 1, 1);
+          continue;
+        }
+        const PassHdr& H = prog.hdrs[prog.hdr_of[i]];
+        maxc = std::max(maxc, H.ncoef);
+        emu_pass(v, n, RB, TB, (uint32_t)amp_bytes, H, prog.ops.data() + H.op0, prog.coef.data() + 2 * H.coef0,
+                 prog.aux.data());
       }
-      if (!prog.plans[i].fused) {
-        cpu_apply_gates(vec, n, &prog.pgates[prog.plans[i].single], 1, 1);
-        continue;
-      }
-      const PassHdr& H = prog.hdrs[prog.hdr_of[i]];
-      maxc = std::max(maxc, H.ncoef);
-      emu_pass(v, n, RB, TB, (uint32_t)amp_bytes, H, prog.ops.data() + H.op0, prog.coef.data() + 2 * H.coef0,
-               prog.aux.data());
     }
     if (stats) {
       int kinds[9] = {0};

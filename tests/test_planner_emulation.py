@@ -162,13 +162,9 @@ def test_low_bits_up_to_the_thread_bits(emu):
     assert rc != 0 and b"low_bits" in emu.emu_last_error()
 
 
-@pytest.mark.parametrize("remap", ["0", "1"])
-def test_wide_controlled_gate_on_high_bits(emu, remap, monkeypatch):
-    """A 9-controlled X as a 10-qubit Unitary on qubits 0..9 (index bits 15..6):
-    more bits above the 4 low bits than a 12-bit tile has room for, so it runs
-    as its own per-gate pass between fused passes."""
-    monkeypatch.setenv("QS_FUSION_REMAP", remap)
-    n = 16
+def wide_controlled(n=16):
+    """A 9-controlled X as a 10-qubit Unitary on qubits 0..9 between two random
+    150-gate runs."""
     m = np.eye(1 << 10, dtype=np.complex128)
     m[-2:, -2:] = [[0, 1], [1, 0]]
     c = workloads.random_mix_circuit(n, 150, 41, workloads.RANDOM_GATE_PROBS_60_40)
@@ -176,9 +172,47 @@ def test_wide_controlled_gate_on_high_bits(emu, remap, monkeypatch):
     c2 = workloads.random_mix_circuit(n, 150, 42, workloads.RANDOM_GATE_PROBS_60_40)
     for g in c2.gates:
         c.append(g)
+    return c
+
+
+@pytest.mark.parametrize("remap", ["0", "1"])
+def test_wide_controlled_gate_on_high_bits(emu, remap, monkeypatch):
+    """A 9-controlled X as a 10-qubit Unitary on qubits 0..9 (index bits 15..6):
+    more bits above the 4 low bits than a 12-bit tile has room for, so it runs
+    as its own per-gate pass between fused passes."""
+    monkeypatch.setenv("QS_FUSION_REMAP", remap)
+    n = 16
+    c = wide_controlled(n)
     out, ref, st = run_both(emu, c, n, C128)
     assert st[0] > st[1]  # one per-gate pass besides the fused ones
     assert np.max(np.abs(out - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["all_types", "flips", "wide_controlled", "dense_multi", "hea20"])
+def test_streamed_chunks_equal_the_whole_program(emu, case, monkeypatch):
+    """qs_sv_apply_gates(fuse=1) uploads and launches the passes in chunks of
+    1, 2, 4, ... while it plans the rest: the rebased chunk programs, with the
+    per-gate passes in between, give the reference kernels' result and the same
+    plan (passes, ops, ops per kind) as the whole program."""
+    from quict_amd.gradient.adjoint import ParameterizedCircuit
+    monkeypatch.setenv("QS_EMU_FUSE", "1")
+    if case == "hea20":  # the 20-qubit 12-layer gradient workload's forward circuit, at 16 qubits
+        pc = ParameterizedCircuit.hea(16, 12)
+        c = Circuit(16)
+        for g, _ in pc.gates(np.random.RandomState(3).uniform(-np.pi, np.pi, pc.nparams)):
+            c.append(g)
+        n = 16
+    else:
+        c, n = {"all_types": (all_types(15, 300, 2), 15), "flips": (flips_and_swaps(16, 500, 6), 16),
+                "wide_controlled": (wide_controlled(16), 16), "dense_multi": (dense_multi(), 16)}[case]
+    out0, ref, st0 = run_both(emu, c, n, C128)
+    monkeypatch.setenv("QS_EMU_STREAM", "1")
+    out, _, st = run_both(emu, c, n, C128)
+    assert np.max(np.abs(out - ref)) < 1e-12
+    assert np.array_equal(st[:11], st0[:11]) and st[1] >= 3  # two chunks or more
+    assert np.max(np.abs(out - out0)) < 1e-13
+    if case == "wide_controlled":
+        assert st[0] > st[1]
 
 
 def test_headline_generator_at_18_qubits(emu):
