@@ -296,6 +296,25 @@ struct TPass {
 // amplitudes apart spread over the banks.
 __device__ __forceinline__ uint32_t swz(uint32_t i) { return i ^ (((i >> 4) ^ (i >> 8)) & 15u); }
 
+// Wave sum by DPP row shifts and row broadcasts (lane moves in the VALU, no
+// LDS round trips): lanes whose source is outside the row add 0; lane 63
+// ends with the sum of all 64 lanes.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, ROWS, 0xf, false),
+                          __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWS, 0xf, false));
+}
+__device__ __forceinline__ double wave_sum_to_lane63(double v) {
+  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8: lane 15 of each row holds the row's sum
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Insert a zero bit at local position p.
 __device__ __forceinline__ uint32_t ins0(uint32_t q, int p) {
   const uint32_t lo = q & ((1u << p) - 1u);
@@ -411,14 +430,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
         }
       }
       if (grad) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        if ((tid & 63u) == 0) red[S.gi * NW + wave] += acc;  // one writer per (slot, wave)
+        acc = wave_sum_to_lane63(acc);
+        if ((tid & 63u) == 63u) red[S.gi * NW + wave] += acc;  // one writer per (slot, wave)
       }
       if (grad2) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc2 += __shfl_down(acc2, o, 64);
-        if ((tid & 63u) == 0) red[S.gi2 * NW + wave] += acc2;
+        acc2 = wave_sum_to_lane63(acc2);
+        if ((tid & 63u) == 63u) red[S.gi2 * NW + wave] += acc2;
       }
       __syncthreads();
     }

@@ -159,7 +159,12 @@ def _rotation_payloads(gtype: GateType, th: np.ndarray):
     angles th, one row per gate: the target blocks of exp(-i th P / 2), of its
     conjugate transpose and of d/dth = -(i/2) P exp(-i th P / 2)
     (QuICT/core/gate/gate.py:963-1096).  Rz records are diagonal."""
-    c, s = np.cos(th / 2), np.sin(th / 2)
+    return _rotation_payloads_cs(gtype, np.cos(th / 2), np.sin(th / 2))
+
+
+def _rotation_payloads_cs(gtype: GateType, c: np.ndarray, s: np.ndarray):
+    """_rotation_payloads from c = cos(th / 2), s = sin(th / 2): every entry is
+    0, +-c, +-s, +-c/2 or +-s/2 (real or imaginary part)."""
     if gtype == GateType.ry:
         fwd = np.stack([c, -s, s, c], 1).astype(np.complex128)
         inv = np.stack([c, s, -s, c], 1).astype(np.complex128)
@@ -169,7 +174,7 @@ def _rotation_payloads(gtype: GateType, th: np.ndarray):
         inv = np.stack([c, 1j * s, 1j * s, c], 1)
         der = np.stack([-s / 2 + 0j, -0.5j * c, -0.5j * c, -s / 2 + 0j], 1)
     else:
-        e0, e1 = np.exp(-0.5j * th), np.exp(0.5j * th)
+        e0, e1 = c - 1j * s, c + 1j * s
         z = np.zeros_like(e0)
         fwd = np.stack([e0, e1], 1)
         inv = np.stack([e1, e0], 1)
@@ -204,7 +209,33 @@ def _template(pc: ParameterizedCircuit):
             fast[gt] = (rows, np.array([pc.ops[j][2] for j in rows]))
     generic = [j for j, (t, _, p, _) in enumerate(pc.ops)
                if p is not None and not any(t == gt for gt in fast)]
-    pc._template = (fwd, inv, der, slot, keep, fast, generic)
+    # every fast payload entry is a * cos(th/2) + b * sin(th/2): per record
+    # array, the (a, b) of each fast row's first 8 matrix words, for one
+    # vectorised refill (columns past a payload get 0, as they hold already)
+    lin = None
+    if fast:
+        rows_all = np.concatenate([fast[gt][0] for gt in fast])
+        params_all = np.concatenate([fast[gt][1] for gt in fast])
+        lin = [rows_all, params_all]
+        for k in range(3):
+            A, B = [], []
+            for gt in fast:
+                nr = len(fast[gt][0])
+                for c_, s_, dst in ((1.0, 0.0, A), (0.0, 1.0, B)):
+                    pay = _rotation_payloads_cs(gt, np.full(nr, c_), np.full(nr, s_))[k]
+                    w = np.zeros((nr, 8))
+                    w[:, : 2 * pay.shape[1]] = np.ascontiguousarray(pay).view(np.float64)
+                    dst.append(w)
+            lin.append((np.concatenate(A), np.concatenate(B)))
+        # the linear form reproduces the compiler's records exactly at the probe angle
+        c0 = np.full(len(rows_all), np.cos(_PROBE_ANGLE / 2))
+        s0 = np.full(len(rows_all), np.sin(_PROBE_ANGLE / 2))
+        for arr, (A, B) in zip((fwd, inv, der), lin[2:]):
+            if not np.array_equal(arr["m"][rows_all, :8], c0[:, None] * A + s0[:, None] * B):
+                fast, lin = {}, None
+                generic = [j for j, (t, _, p, _) in enumerate(pc.ops) if p is not None]
+                break
+    pc._template = (fwd, inv, der, slot, keep, fast, generic, lin)
     return pc._template
 
 
@@ -214,7 +245,7 @@ def _records(pc: ParameterizedCircuit, theta: np.ndarray, reuse: bool = False):
     other parameterised gates go through the generic compiler.  reuse: refill
     the circuit's own working copies instead of new arrays (valid until the
     next such call; for callers that hand them to one native call)."""
-    fwd0, inv0, der0, slot, keep0, fast, generic = _template(pc)
+    fwd0, inv0, der0, slot, keep0, fast, generic, lin = _template(pc)
     if reuse:
         if pc._work is None or pc._work[0] is not fwd0:
             pc._work = (fwd0, fwd0.copy(), inv0.copy(), der0.copy())
@@ -222,10 +253,11 @@ def _records(pc: ParameterizedCircuit, theta: np.ndarray, reuse: bool = False):
     else:
         fwd, inv, der = fwd0.copy(), inv0.copy(), der0.copy()
     keep = list(keep0)
-    for gt, (rows, params) in fast.items():
-        f, i, d = _rotation_payloads(gt, np.asarray(theta, dtype=np.float64)[params])
-        for arr, pay in ((fwd, f), (inv, i), (der, d)):
-            arr["m"][rows, : 2 * pay.shape[1]] = pay.view(np.float64)
+    if lin is not None:
+        half = np.asarray(theta, dtype=np.float64)[lin[1]] / 2
+        c, s = np.cos(half)[:, None], np.sin(half)[:, None]
+        for arr, (A, B) in zip((fwd, inv, der), lin[2:]):
+            arr["m"][lin[0], :8] = c * A + s * B
     if generic:
         for j in generic:
             g = pc.gate(j, theta)
