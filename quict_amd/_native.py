@@ -101,7 +101,7 @@ SIGNATURES = {
     "qs_sv_apply_pauli_sum": (_I, [_P, _P, _P, _I, _I, _P, _I, _P]),
     "qs_sv_inner": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D), _P]),
     "qs_adjoint_gradient": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _I, _P]),
-    "qs_adjoint_gradient_pauli": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _I, _P, _P]),
+    "qs_adjoint_gradient_pauli": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _P, _I, _P, _P, _P, _I, _P, _P]),
     "qs_qnn_batch": (_I, [_P, _I, _I, _P, _I, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "qs_qnn_batch_programs": (_I, [_P, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }

@@ -276,8 +276,9 @@ struct TStep {
   int32_t p0, p1;        // tile-local positions of target 0 (matrix MSB) and target 1
   int32_t gi;            // parameterised: accumulator slot in the pass, else -1
   uint32_t cmask, cval;  // tile-local control bits and the values they need
-  int32_t twice;         // one target: a second step on the same bit and controls follows
-                         // (G^dagger in g[8..15], dG in d[8..15]), one LDS round trip for both
+  int32_t twice;         // one target: the next gate of the pass, on the same bit and controls,
+                         // shares the step (its G^dagger in g[8..15], dG in d[8..15]): one LDS
+                         // round trip for both
   int32_t gi2;           // its accumulator slot, else -1
   double g[32];          // G^dagger, (re, im), 2^nt x 2^nt
   double d[32];          // dG/dtheta
@@ -286,7 +287,7 @@ struct TStep {
 struct TPass {
   int32_t ubits[AT_MAX_BITS];  // amplitude bits of the tile, ascending (tile-local bit b = ubits[b])
   int32_t nb;
-  int32_t s0, ns;              // steps [s0, s0 + ns) of the step table, in application order
+  int32_t s0, ns;              // steps [s0, s0 + ns) of the step table, in circuit order
   int32_t ngrad;
   int32_t row[AT_MAX_GRADS];   // partial row of accumulator slot k
 };
@@ -352,28 +353,19 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
       sl[swz(i)] = ld(lam + a);
     }
     __syncthreads();
-    for (int k = 0; k < P.ns; ++k) {
+    for (int k = P.ns - 1; k >= 0; --k) {  // the pass's steps are in circuit order
       const TStep& S = steps[P.s0 + k];
       double acc = 0.0;
       const bool grad = S.gi >= 0;
       double acc2 = 0.0;
       const bool twice = S.twice != 0, grad2 = twice && S.gi2 >= 0;
       if (S.nt == 1) {
-        const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
         for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
           const uint32_t i0 = ins0(q, S.p0), i1 = i0 | (1u << S.p0);
           if ((i0 & S.cmask) != S.cval) continue;
           const uint32_t j0 = swz(i0), j1 = swz(i1);
-          const cx<R> a0 = sp[j0], a1 = sp[j1], b0 = sl[j0], b1 = sl[j1];
-          cx<R> pa = cfma(mul(g00, a0), g01, a1), pb = cfma(mul(g10, a0), g11, a1);
-          cx<R> la = cfma(mul(g00, b0), g01, b1), lb = cfma(mul(g10, b0), g11, b1);
-          if (grad) {
-            const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), pa), cget<R>(S.d, 1), pb);
-            const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), pa), cget<R>(S.d, 3), pb);
-            acc += (double)b0.re * (double)m0.re + (double)b0.im * (double)m0.im + (double)b1.re * (double)m1.re +
-                   (double)b1.im * (double)m1.im;
-          }
-          if (twice) {  // the following step, on the same pair
+          cx<R> pa = sp[j0], pb = sp[j1], la = sl[j0], lb = sl[j1];
+          if (twice) {  // the later gate of the pair (g[8..15]) is undone first
             const cx<R> h00 = cget<R>(S.g, 4), h01 = cget<R>(S.g, 5), h10 = cget<R>(S.g, 6), h11 = cget<R>(S.g, 7);
             const cx<R> qa = cfma(mul(h00, pa), h01, pb), qb = cfma(mul(h10, pa), h11, pb);
             if (grad2) {
@@ -388,10 +380,18 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
             la = ma;
             lb = mb;
           }
-          sp[j0] = pa;
-          sp[j1] = pb;
-          sl[j0] = la;
-          sl[j1] = lb;
+          const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
+          const cx<R> qa = cfma(mul(g00, pa), g01, pb), qb = cfma(mul(g10, pa), g11, pb);
+          if (grad) {
+            const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), qa), cget<R>(S.d, 1), qb);
+            const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), qa), cget<R>(S.d, 3), qb);
+            acc += (double)la.re * (double)m0.re + (double)la.im * (double)m0.im + (double)lb.re * (double)m1.re +
+                   (double)lb.im * (double)m1.im;
+          }
+          sl[j0] = cfma(mul(g00, la), g01, lb);
+          sl[j1] = cfma(mul(g10, la), g11, lb);
+          sp[j0] = qa;
+          sp[j1] = qb;
         }
       } else {
         const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
@@ -455,6 +455,88 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
   }
 }
 
+// conj(m[i]): entries of a conjugate transpose.
+template <typename R> __device__ __forceinline__ cx<R> cgetc(const double* m, int i) {
+  return mk<R>((R)m[2 * i], -(R)m[2 * i + 1]);
+}
+
+// The forward state from the sweep's own plan: the tile passes (planned in
+// circuit order; the sweep runs them last first, each step last first) run
+// in order, each applying its steps in order with G = (G^dagger)^dagger (the
+// conjugate transpose of the step's matrix), on psi alone.  The grouping only
+// exchanges commuting gates, so both directions compute the circuit's U.
+template <typename R>
+__global__ __launch_bounds__(ADJ_THREADS) void k_fwd_tile(cx<R>* __restrict__ psi, uint64_t ntiles, const TPass P,
+                                                          const TStep* __restrict__ steps) {
+  extern __shared__ __align__(16) char at_lds[];
+  const int nb = P.nb;
+  const uint32_t NA = 1u << nb;
+  cx<R>* sp = reinterpret_cast<cx<R>*>(at_lds);
+  const uint32_t tid = threadIdx.x;
+  uint64_t dlo = 0;
+  for (int b = 0; b < nb && b < 8; ++b)
+    if ((tid >> b) & 1u) dlo |= 1ull << P.ubits[b];
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = insert_zeros(t, P.ubits, nb);
+    __syncthreads();  // the previous tile's stores have read the LDS image
+    for (uint32_t i = tid; i < NA; i += ADJ_THREADS) {
+      uint64_t a = base | dlo;
+      for (int b = 8; b < nb; ++b)
+        if ((i >> b) & 1u) a |= 1ull << P.ubits[b];
+      sp[swz(i)] = ld(psi + a);
+    }
+    __syncthreads();
+    for (int k = 0; k < P.ns; ++k) {
+      const TStep& S = steps[P.s0 + k];
+      if (S.nt == 1) {
+        for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
+          const uint32_t i0 = ins0(q, S.p0), i1 = i0 | (1u << S.p0);
+          if ((i0 & S.cmask) != S.cval) continue;
+          const uint32_t j0 = swz(i0), j1 = swz(i1);
+          cx<R> a0 = sp[j0], a1 = sp[j1];
+          const cx<R> b0 = cfma(mul(cgetc<R>(S.g, 0), a0), cgetc<R>(S.g, 2), a1);
+          const cx<R> b1 = cfma(mul(cgetc<R>(S.g, 1), a0), cgetc<R>(S.g, 3), a1);
+          a0 = b0;
+          a1 = b1;
+          if (S.twice) {  // the later gate of the pair
+            const cx<R> c0 = cfma(mul(cgetc<R>(S.g, 4), a0), cgetc<R>(S.g, 6), a1);
+            const cx<R> c1 = cfma(mul(cgetc<R>(S.g, 5), a0), cgetc<R>(S.g, 7), a1);
+            a0 = c0;
+            a1 = c1;
+          }
+          sp[j0] = a0;
+          sp[j1] = a1;
+        }
+      } else {
+        const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
+        for (uint32_t q = tid; q < NA / 4; q += ADJ_THREADS) {
+          const uint32_t i = ins0(ins0(q, plo), phi);
+          if ((i & S.cmask) != S.cval) continue;
+          const uint32_t idx[4] = {swz(i), swz(i | (1u << S.p1)), swz(i | (1u << S.p0)),
+                                   swz(i | (1u << S.p0) | (1u << S.p1))};
+          cx<R> a[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[r] = sp[idx[r]];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            cx<R> x = mul(cgetc<R>(S.g, r), a[0]);
+#pragma unroll
+            for (int c = 1; c < 4; ++c) x = cfma(x, cgetc<R>(S.g, 4 * c + r), a[c]);
+            sp[idx[r]] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = tid; i < NA; i += ADJ_THREADS) {
+      uint64_t a = base | dlo;
+      for (int b = 8; b < nb; ++b)
+        if ((i >> b) & 1u) a |= 1ull << P.ubits[b];
+      st(psi + a, sp[swz(i)]);
+    }
+  }
+}
+
 // sums[k] = sum_b partial[k * stride + b], b < nb[k]
 __global__ void k_adj_reduce(const double* __restrict__ partial, const int* __restrict__ nb, int stride, int nsteps,
                              double* __restrict__ sums) {
@@ -471,9 +553,10 @@ __global__ void k_adj_reduce(const double* __restrict__ partial, const int* __re
 }
 
 // Dense 2^t x 2^t form of a record with t <= 2 (controls kept separately).
-static void record_dense(const qs_gate& g, std::vector<double>& M) {
+// Dense (re, im) matrix of a record with at most 2 targets into M[2 * 4^t].
+static void record_dense(const qs_gate& g, double* M) {
   const int t = g.ntargets, D = 1 << t;
-  M.assign(2 * D * D, 0.0);
+  for (int i = 0; i < 2 * D * D; ++i) M[i] = 0.0;
   auto set = [&](int i, int j, double re, double im) {
     M[2 * (i * D + j)] = re;
     M[2 * (i * D + j) + 1] = im;
@@ -502,6 +585,11 @@ static void record_dense(const qs_gate& g, std::vector<double>& M) {
   }
 }
 
+static void record_dense(const qs_gate& g, std::vector<double>& M) {
+  M.assign(2 << (2 * g.ntargets), 0.0);
+  record_dense(g, M.data());
+}
+
 // terms != nullptr: lam is computed here as H psi (and energy = <psi|H psi>)
 // after the sweep is planned, so the planning overlaps the forward passes
 // still running on the stream, and the energy comes back with the gradient
@@ -509,14 +597,17 @@ static void record_dense(const qs_gate& g, std::vector<double>& M) {
 template <typename R>
 static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* gates, int ngates,
                      const qs_gate* dgates, const int32_t* slot, double* grads, int nslots, hipStream_t s,
-                     const qs_pauli_term* terms = nullptr, int nterms = 0, double* energy = nullptr) {
+                     const qs_pauli_term* terms = nullptr, int nterms = 0, double* energy = nullptr,
+                     const qs_gate* fwd = nullptr) {
   int rc = QS_OK;
   std::vector<int> nb;
   std::vector<int> step_slot;
-  // Plan: runs of consecutive steps (backward order) become one pass: tile
-  // passes in LDS (default: up to AT_MAX_STEPS steps whose bits fit in a tile of
-  // `tile_bits` amplitude bits with the AT_LOW lowest ones), or register passes
-  // spanning <= 3 amplitude bits (QS_ADJ_TILE=0); other gates run as single steps.
+  // Plan: runs of consecutive steps become one pass: tile passes in LDS
+  // (default: up to AT_MAX_STEPS steps whose bits fit in a tile of `tile_bits`
+  // amplitude bits with the AT_LOW lowest ones; planned in circuit order, the
+  // sweep takes them last first), or register passes spanning <= 3 amplitude
+  // bits (QS_ADJ_TILE=0; planned in backward order); other gates run as
+  // single steps.
   struct Item {
     int multi;   // 1: register pass index into passes, 2: tile pass index into tpasses, 0: single gate index
     int idx;
@@ -533,11 +624,61 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   const uint64_t lowmask = (1ull << std::min(n, AT_LOW)) - 1;
   std::vector<int> cur;  // gate indices of the open pass, in application order
   uint64_t curbits = 0;
-  auto gbits = [&](int j) {
+  std::vector<uint64_t> gmask(ngates);
+  for (int j = 0; j < ngates; ++j) {
     uint64_t m = 0;
     for (int i = 0; i < gates[j].ncontrols + gates[j].ntargets; ++i) m |= 1ull << gates[j].bits[i];
-    return m;
-  };
+    gmask[j] = m;
+  }
+  auto gbits = [&](int j) { return gmask[j]; };
+  ts.reserve(ngates);
+  const int dt = sizeof(R) == 8 ? QS_C128 : QS_C64;
+  if (fwd && !tile) {
+    // the forward state by the fused passes, before this call takes a program
+    // slot (their uploads cycle through the slots)
+    rc = qs_sv_init(ctx, psi, n, dt, 0, s);
+    if (!rc) rc = apply_gates_fused(ctx, psi, n, dt, fwd, ngates, false, nullptr, s);
+    if (rc) return rc;
+  }
+  // Every device buffer of the call lives in one pinned program slot, sized
+  // before planning (at most one step or register-pass gate per gate, one
+  // partial row per parameterised gate): the descriptors (pass steps, block
+  // counts; uploaded from pinned memory, since a copy from pageable memory
+  // would wait for forward passes still queued), the partial rows, sums and
+  // the energy's partials (nothing else the call runs can move or overwrite
+  // them: per-gate passes use the context's scratch); the slot's host side
+  // takes the results back.
+  int nparam = 0;
+  for (int j = 0; j < ngates; ++j) nparam += slot[j] >= 0;
+  const uint64_t rows = std::max(nparam, 1);
+  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+  const uint64_t ts_off = al((tile ? 0 : (uint64_t)ngates) * sizeof(MGate));
+  const uint64_t nb_off = al(ts_off + (tile ? (uint64_t)ngates : 0) * sizeof(TStep));
+  const uint64_t up = nb_off + rows * sizeof(int), res_off = al(up);
+  const uint64_t part_off = al(up), sums_off = al(part_off + rows * ADJ_MAX_BLOCKS * sizeof(double));
+  const uint64_t ip_off = al(sums_off + rows * sizeof(double));
+  const uint64_t dev_end = ip_off + (2 * (uint64_t)inner_partial_blocks() + 64) * sizeof(double);
+  qs_ctx::ProgSlot* pslot = nullptr;
+  rc = acquire_prog_slot(ctx, std::max<uint64_t>(dev_end, res_off + (rows + 2) * sizeof(double)), &pslot);
+  if (rc) return rc;
+  struct SlotGuard {  // the slot is busy until everything enqueued below has run
+    qs_ctx::ProgSlot* sl;
+    hipStream_t st;
+    ~SlotGuard() {
+      (void)hipEventRecord(sl->done, st);
+      sl->pending = true;
+    }
+  } guard{pslot, s};
+  char* hs = pslot->host;
+  char* ds = (char*)pslot->dev;
+  const MGate* d_mg = (const MGate*)ds;
+  const TStep* d_ts = (const TStep*)(ds + ts_off);
+  const int* d_nb = (const int*)(ds + nb_off);
+  double* partial = (double*)(ds + part_off);
+  double* sums = (double*)(ds + sums_off);
+  double* ipart = (double*)(ds + ip_off);
+  double* ires = ipart + 2 * inner_partial_blocks();
+  double* h_res = (double*)(hs + res_off);  // grads rows, then the energy (re, im)
   auto eligible = [&](int j) {
     const qs_gate& g = gates[j];
     if (g.ntargets > 2) return false;
@@ -575,7 +716,8 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
         S.cmask |= 1u << local(g.bits[i]);
         S.cval |= 1u << local(g.bits[i]);
       }
-      std::vector<double> M, Dm;
+      double M[32], Dm[32];
+      const int msz = 2 << (2 * g.ntargets);
       record_dense(g, M);
       if (slot[j] >= 0) record_dense(dgates[j], Dm);
       // a one-target step on the bit and controls of the previous one-target
@@ -584,9 +726,9 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
       if (S.nt == 1 && prev && prev->nt == 1 && !prev->twice && prev->p0 == S.p0 && prev->cmask == S.cmask &&
           prev->cval == S.cval) {
         prev->twice = 1;
-        for (size_t i = 0; i < M.size(); ++i) prev->g[8 + i] = M[i];
+        for (int i = 0; i < msz; ++i) prev->g[8 + i] = M[i];
         if (slot[j] >= 0) {
-          for (size_t i = 0; i < Dm.size(); ++i) prev->d[8 + i] = Dm[i];
+          for (int i = 0; i < msz; ++i) prev->d[8 + i] = Dm[i];
           prev->gi2 = P.ngrad;
           P.row[P.ngrad++] = (int)nb.size();
           nb.push_back(blocks);
@@ -594,9 +736,9 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
         }
         continue;
       }
-      for (size_t i = 0; i < M.size(); ++i) S.g[i] = M[i];
+      for (int i = 0; i < msz; ++i) S.g[i] = M[i];
       if (slot[j] >= 0) {
-        for (size_t i = 0; i < Dm.size(); ++i) S.d[i] = Dm[i];
+        for (int i = 0; i < msz; ++i) S.d[i] = Dm[i];
         S.gi = P.ngrad;
         P.row[P.ngrad++] = (int)nb.size();
         nb.push_back(blocks);
@@ -657,14 +799,53 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     curbits = 0;
   };
   std::vector<int> single_row(ngates, -1);
+  // With the forward records and tile passes, the forward state is computed
+  // while the rest is planned: |0...0>, then the items as they are planned
+  // (circuit order), their steps uploaded in chunks of 1, 2, 4, ... 16 passes.
+  const bool stream_fwd = fwd && tile;
+  size_t ts_up = 0, items_fwd = 0;
+  int fwd_chunk = 1;
+  if (stream_fwd) {
+    rc = qs_sv_init(ctx, psi, n, dt, 0, s);
+    if (rc) return rc;
+  }
+  auto forward_items = [&](bool all) -> int {
+    if (!stream_fwd) return QS_OK;
+    int pending = 0;
+    for (size_t i = items_fwd; i < items.size(); ++i) pending += items[i].multi == 2;
+    if (!all && pending < fwd_chunk) return QS_OK;
+    if (ts.size() > ts_up) {
+      const size_t b = ts_up * sizeof(TStep), e = ts.size() * sizeof(TStep);
+      std::memcpy(hs + ts_off + b, (const char*)ts.data() + b, e - b);
+      QS_CHECK_HIP(hipMemcpyAsync(ds + ts_off + b, hs + ts_off + b, e - b, hipMemcpyHostToDevice, s));
+      ts_up = ts.size();
+    }
+    for (; items_fwd < items.size(); ++items_fwd) {
+      const Item& it = items[items_fwd];
+      if (it.multi == 2) {
+        const TPass& P = tpasses[it.idx];
+        const uint64_t ntiles = 1ull << (n - P.nb);
+        const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
+        hipLaunchKernelGGL(k_fwd_tile<R>, dim3(blocks), dim3(ADJ_THREADS), ((size_t)1 << P.nb) * sizeof(cx<R>), s,
+                           psi, ntiles, P, d_ts);
+        QS_CHECK_HIP(hipGetLastError());
+      } else {
+        const int r = apply_gate(ctx, psi, n, dt, &fwd[it.idx], s);
+        if (r) return r;
+      }
+    }
+    fwd_chunk = std::min(2 * fwd_chunk, 16);
+    return QS_OK;
+  };
   // Tile passes group the steps like the forward fusion planner: within a run
-  // of eligible steps (backward order), a step may join the current pass ahead
-  // of steps still waiting if it shares no bit with them (disjoint gates
-  // commute, so the gradient of the same function comes out), fewest new tile
-  // bits first; the pass applies its steps in backward order.
+  // of eligible gates (circuit order), a gate may join the current pass ahead
+  // of gates still waiting if it shares no bit with them (disjoint gates
+  // commute, so the same U and the gradient of the same function come out),
+  // fewest new tile bits first.  The forward applies a pass's steps in order,
+  // the sweep undoes them last first.
   std::vector<int> run;
   const uint64_t all_bits = n >= 64 ? ~0ull : (1ull << n) - 1;
-  auto plan_run = [&]() {
+  auto plan_run = [&]() -> int {
     std::vector<char> done(run.size(), 0);
     size_t left = run.size(), first = 0;  // steps before `first` are all placed
     while (left) {
@@ -705,15 +886,19 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
           --left;
         }
       flush_tile();
+      const int r = forward_items(false);
+      if (r) return r;
     }
     run.clear();
+    return QS_OK;
   };
-  for (int j = ngates - 1; j >= 0; --j) {
+  for (int jj = 0; jj < ngates; ++jj) {
+    const int j = tile ? jj : ngates - 1 - jj;
     if (tile && eligible(j)) {
       run.push_back(j);
       continue;
     }
-    if (tile) plan_run();
+    if (tile && (rc = plan_run())) return rc;
     if (!eligible(j)) {
       flush();
       if (slot[j] >= 0) {
@@ -728,6 +913,7 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
         step_slot.push_back(slot[j]);
       }
       items.push_back({0, j});
+      if ((rc = forward_items(false))) return rc;
       continue;
     }
     const uint64_t b = gbits(j);
@@ -735,45 +921,24 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     cur.push_back(j);
     curbits |= b;
   }
-  if (tile) plan_run();
+  if (tile && (rc = plan_run())) return rc;
   flush();
-  // scratch: partial rows, sums, the energy's partials; descriptors (pass
-  // steps, block counts) go up in one copy from a pinned slot (a copy from
-  // pageable memory would wait for the forward passes still queued), which
-  // also takes the results back
-  const uint64_t rows = std::max<size_t>(nb.size(), 1);
-  const uint64_t need = rows * ADJ_MAX_BLOCKS * sizeof(double) + rows * sizeof(double) + 512 +
-                        (2 * (uint64_t)inner_partial_blocks() + 64) * sizeof(double);
-  rc = ensure_scratch(ctx, need);
-  if (rc) return rc;
-  double* partial = (double*)ctx->scratch;
-  double* sums = partial + rows * ADJ_MAX_BLOCKS;
-  double* ipart = (double*)(((uintptr_t)(sums + rows) + 255) & ~(uintptr_t)255);
-  double* ires = ipart + 2 * inner_partial_blocks();
-  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
-  const uint64_t ts_off = al(mg.size() * sizeof(MGate)), nb_off = al(ts_off + ts.size() * sizeof(TStep));
-  const uint64_t up = nb_off + rows * sizeof(int), res_off = al(up);
-  qs_ctx::ProgSlot* pslot = nullptr;
-  rc = acquire_prog_slot(ctx, res_off + (rows + 2) * sizeof(double), &pslot);
-  if (rc) return rc;
-  struct SlotGuard {  // the slot is busy until everything enqueued below has run
-    qs_ctx::ProgSlot* sl;
-    hipStream_t st;
-    ~SlotGuard() {
-      (void)hipEventRecord(sl->done, st);
-      sl->pending = true;
-    }
-  } guard{pslot, s};
-  char* hs = pslot->host;
+  if ((rc = forward_items(true))) return rc;
+  if ((int)nb.size() != nparam || mg.size() > (size_t)(tile ? 0 : ngates) || ts.size() > (size_t)ngates) {
+    set_error("adjoint gradient: internal error: plan larger than its buffers");
+    return QS_ERR_DEVICE;
+  }
+  // the rest of the descriptors: steps not yet uploaded, register-pass gates, block counts
   if (!mg.empty()) std::memcpy(hs, mg.data(), mg.size() * sizeof(MGate));
-  if (!ts.empty()) std::memcpy(hs + ts_off, ts.data(), ts.size() * sizeof(TStep));
+  if (ts.size() > ts_up) std::memcpy(hs + ts_off + ts_up * sizeof(TStep), ts.data() + ts_up, (ts.size() - ts_up) * sizeof(TStep));
   if (!nb.empty()) std::memcpy(hs + nb_off, nb.data(), nb.size() * sizeof(int));
-  QS_CHECK_HIP(hipMemcpyAsync(pslot->dev, hs, up, hipMemcpyHostToDevice, s));
-  const MGate* d_mg = (const MGate*)pslot->dev;
-  const TStep* d_ts = (const TStep*)((char*)pslot->dev + ts_off);
-  const int* d_nb = (const int*)((char*)pslot->dev + nb_off);
-  double* h_res = (double*)(hs + res_off);  // grads rows, then the energy (re, im)
-  const int dt = sizeof(R) == 8 ? QS_C128 : QS_C64;
+  if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(ds, hs, mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
+  if (ts.size() > ts_up) {
+    const size_t b = ts_off + ts_up * sizeof(TStep);
+    QS_CHECK_HIP(hipMemcpyAsync(ds + b, hs + b, (ts.size() - ts_up) * sizeof(TStep), hipMemcpyHostToDevice, s));
+  }
+  QS_CHECK_HIP(hipMemcpyAsync(ds + nb_off, hs + nb_off, rows * sizeof(int), hipMemcpyHostToDevice, s));
+  if (tile) std::reverse(items.begin(), items.end());  // the sweep: last pass first
   if (energy) {
     rc = pauli_sum_async(ctx, psi, lam, n, dt, terms, nterms, s);
     if (rc) return rc;
@@ -909,8 +1074,9 @@ int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, con
 }
 
 int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_pauli_term* terms,
-                              int nterms, const qs_gate* gates, int ngates, const qs_gate* dgates,
-                              const int32_t* param_slot, double* grads, int nslots, double* energy, void* stream) {
+                              int nterms, const qs_gate* fwd_gates, const qs_gate* gates, int ngates,
+                              const qs_gate* dgates, const int32_t* param_slot, double* grads, int nslots,
+                              double* energy, void* stream) {
   int rc = check_state_args(psi, n, dtype);
   if (rc) return rc;
   QS_REQUIRE(ctx && lam && lam != psi && energy, "bad state buffers or energy pointer");
@@ -921,6 +1087,10 @@ int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtyp
   for (int j = 0; j < ngates; ++j) {
     rc = validate_gate(&gates[j], n);
     if (rc) return rc;
+    if (fwd_gates) {
+      rc = validate_gate(&fwd_gates[j], n);
+      if (rc) return rc;
+    }
     if (param_slot[j] >= 0) {
       QS_REQUIRE(dgates != nullptr, "derivative records missing");
       QS_REQUIRE(param_slot[j] < nslots, "parameter slot out of range");
@@ -930,9 +1100,9 @@ int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtyp
   }
   if (dtype == QS_C128)
     return adjoint_t<double>(ctx, (cx<double>*)psi, (cx<double>*)lam, n, gates, ngates, dgates, param_slot, grads,
-                             nslots, as_stream(stream), terms, nterms, energy);
+                             nslots, as_stream(stream), terms, nterms, energy, fwd_gates);
   return adjoint_t<float>(ctx, (cx<float>*)psi, (cx<float>*)lam, n, gates, ngates, dgates, param_slot, grads, nslots,
-                          as_stream(stream), terms, nterms, energy);
+                          as_stream(stream), terms, nterms, energy, fwd_gates);
 }
 
 }  // extern "C"
