@@ -265,11 +265,10 @@ int qs_adjoint_gradient(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, con
    running on the stream).  fwd_gates given (ngates records, fwd_gates[j] the
    gate whose inverse is gates[j]): psi is set to |0...0> and the forward
    state is computed here from the sweep's own plan: its LDS tile passes run
-   in circuit order with the conjugate transposes of the gates they hold, each
-   launched while the rest is planned (gates outside the passes run from
-   fwd_gates); the sweep then takes the same passes last first.  No
-   counterpart in the reference, whose gradient code lives in the external
-   QuICT_ml. */
+   last first with the conjugate transposes of the gates they hold (gates
+   outside the passes run from fwd_gates): one plan serves both directions.
+   No counterpart in the reference, whose gradient code lives in the
+   external QuICT_ml. */
 int qs_adjoint_gradient_pauli(qs_ctx* ctx, void* psi, void* lam, int n, int dtype, const qs_pauli_term* terms,
                               int nterms, const qs_gate* fwd_gates, const qs_gate* gates, int ngates,
                               const qs_gate* dgates, const int32_t* param_slot, double* grads, int nslots,

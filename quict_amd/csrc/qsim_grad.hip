@@ -276,9 +276,9 @@ struct TStep {
   int32_t p0, p1;        // tile-local positions of target 0 (matrix MSB) and target 1
   int32_t gi;            // parameterised: accumulator slot in the pass, else -1
   uint32_t cmask, cval;  // tile-local control bits and the values they need
-  int32_t twice;         // one target: the next gate of the pass, on the same bit and controls,
-                         // shares the step (its G^dagger in g[8..15], dG in d[8..15]): one LDS
-                         // round trip for both
+  int32_t twice;         // one target: the next step of the pass (the gate before, in circuit
+                         // order), on the same bit and controls, shares the step (its G^dagger
+                         // in g[8..15], dG in d[8..15]): one LDS round trip for both
   int32_t gi2;           // its accumulator slot, else -1
   double g[32];          // G^dagger, (re, im), 2^nt x 2^nt
   double d[32];          // dG/dtheta
@@ -287,7 +287,7 @@ struct TStep {
 struct TPass {
   int32_t ubits[AT_MAX_BITS];  // amplitude bits of the tile, ascending (tile-local bit b = ubits[b])
   int32_t nb;
-  int32_t s0, ns;              // steps [s0, s0 + ns) of the step table, in circuit order
+  int32_t s0, ns;              // steps [s0, s0 + ns) of the step table, in sweep (backward) order
   int32_t ngrad;
   int32_t row[AT_MAX_GRADS];   // partial row of accumulator slot k
 };
@@ -353,19 +353,28 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
       sl[swz(i)] = ld(lam + a);
     }
     __syncthreads();
-    for (int k = P.ns - 1; k >= 0; --k) {  // the pass's steps are in circuit order
+    for (int k = 0; k < P.ns; ++k) {
       const TStep& S = steps[P.s0 + k];
       double acc = 0.0;
       const bool grad = S.gi >= 0;
       double acc2 = 0.0;
       const bool twice = S.twice != 0, grad2 = twice && S.gi2 >= 0;
       if (S.nt == 1) {
+        const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
         for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
           const uint32_t i0 = ins0(q, S.p0), i1 = i0 | (1u << S.p0);
           if ((i0 & S.cmask) != S.cval) continue;
           const uint32_t j0 = swz(i0), j1 = swz(i1);
-          cx<R> pa = sp[j0], pb = sp[j1], la = sl[j0], lb = sl[j1];
-          if (twice) {  // the later gate of the pair (g[8..15]) is undone first
+          const cx<R> a0 = sp[j0], a1 = sp[j1], b0 = sl[j0], b1 = sl[j1];
+          cx<R> pa = cfma(mul(g00, a0), g01, a1), pb = cfma(mul(g10, a0), g11, a1);
+          cx<R> la = cfma(mul(g00, b0), g01, b1), lb = cfma(mul(g10, b0), g11, b1);
+          if (grad) {
+            const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), pa), cget<R>(S.d, 1), pb);
+            const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), pa), cget<R>(S.d, 3), pb);
+            acc += (double)b0.re * (double)m0.re + (double)b0.im * (double)m0.im + (double)b1.re * (double)m1.re +
+                   (double)b1.im * (double)m1.im;
+          }
+          if (twice) {  // the following step, on the same pair
             const cx<R> h00 = cget<R>(S.g, 4), h01 = cget<R>(S.g, 5), h10 = cget<R>(S.g, 6), h11 = cget<R>(S.g, 7);
             const cx<R> qa = cfma(mul(h00, pa), h01, pb), qb = cfma(mul(h10, pa), h11, pb);
             if (grad2) {
@@ -380,18 +389,10 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_tile(cx<R>* __restrict__ ps
             la = ma;
             lb = mb;
           }
-          const cx<R> g00 = cget<R>(S.g, 0), g01 = cget<R>(S.g, 1), g10 = cget<R>(S.g, 2), g11 = cget<R>(S.g, 3);
-          const cx<R> qa = cfma(mul(g00, pa), g01, pb), qb = cfma(mul(g10, pa), g11, pb);
-          if (grad) {
-            const cx<R> m0 = cfma(mul(cget<R>(S.d, 0), qa), cget<R>(S.d, 1), qb);
-            const cx<R> m1 = cfma(mul(cget<R>(S.d, 2), qa), cget<R>(S.d, 3), qb);
-            acc += (double)la.re * (double)m0.re + (double)la.im * (double)m0.im + (double)lb.re * (double)m1.re +
-                   (double)lb.im * (double)m1.im;
-          }
-          sl[j0] = cfma(mul(g00, la), g01, lb);
-          sl[j1] = cfma(mul(g10, la), g11, lb);
-          sp[j0] = qa;
-          sp[j1] = qb;
+          sp[j0] = pa;
+          sp[j1] = pb;
+          sl[j0] = la;
+          sl[j1] = lb;
         }
       } else {
         const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
@@ -461,10 +462,10 @@ template <typename R> __device__ __forceinline__ cx<R> cgetc(const double* m, in
 }
 
 // The forward state from the sweep's own plan: the tile passes (planned in
-// circuit order; the sweep runs them last first, each step last first) run
-// in order, each applying its steps in order with G = (G^dagger)^dagger (the
-// conjugate transpose of the step's matrix), on psi alone.  The grouping only
-// exchanges commuting gates, so both directions compute the circuit's U.
+// backward order, as the sweep runs them) run last first, each applying its
+// steps last first with G = (G^dagger)^dagger (the conjugate transpose of the
+// step's matrix), on psi alone.  The grouping only exchanges commuting gates,
+// so both directions compute the circuit's U.
 template <typename R>
 __global__ __launch_bounds__(ADJ_THREADS) void k_fwd_tile(cx<R>* __restrict__ psi, uint64_t ntiles, const TPass P,
                                                           const TStep* __restrict__ steps) {
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_fwd_tile(cx<R>* __restrict__ ps
       sp[swz(i)] = ld(psi + a);
     }
     __syncthreads();
-    for (int k = 0; k < P.ns; ++k) {
+    for (int k = P.ns - 1; k >= 0; --k) {  // the pass's steps are in sweep order
       const TStep& S = steps[P.s0 + k];
       if (S.nt == 1) {
         for (uint32_t q = tid; q < NA / 2; q += ADJ_THREADS) {
@@ -494,18 +495,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_fwd_tile(cx<R>* __restrict__ ps
           if ((i0 & S.cmask) != S.cval) continue;
           const uint32_t j0 = swz(i0), j1 = swz(i1);
           cx<R> a0 = sp[j0], a1 = sp[j1];
-          const cx<R> b0 = cfma(mul(cgetc<R>(S.g, 0), a0), cgetc<R>(S.g, 2), a1);
-          const cx<R> b1 = cfma(mul(cgetc<R>(S.g, 1), a0), cgetc<R>(S.g, 3), a1);
-          a0 = b0;
-          a1 = b1;
-          if (S.twice) {  // the later gate of the pair
+          if (S.twice) {  // g[8..15] holds the earlier gate of the pair
             const cx<R> c0 = cfma(mul(cgetc<R>(S.g, 4), a0), cgetc<R>(S.g, 6), a1);
             const cx<R> c1 = cfma(mul(cgetc<R>(S.g, 5), a0), cgetc<R>(S.g, 7), a1);
             a0 = c0;
             a1 = c1;
           }
-          sp[j0] = a0;
-          sp[j1] = a1;
+          sp[j0] = cfma(mul(cgetc<R>(S.g, 0), a0), cgetc<R>(S.g, 2), a1);
+          sp[j1] = cfma(mul(cgetc<R>(S.g, 1), a0), cgetc<R>(S.g, 3), a1);
         }
       } else {
         const int plo = S.p0 < S.p1 ? S.p0 : S.p1, phi = S.p0 < S.p1 ? S.p1 : S.p0;
@@ -602,11 +599,10 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   int rc = QS_OK;
   std::vector<int> nb;
   std::vector<int> step_slot;
-  // Plan: runs of consecutive steps become one pass: tile passes in LDS
-  // (default: up to AT_MAX_STEPS steps whose bits fit in a tile of `tile_bits`
-  // amplitude bits with the AT_LOW lowest ones; planned in circuit order, the
-  // sweep takes them last first), or register passes spanning <= 3 amplitude
-  // bits (QS_ADJ_TILE=0; planned in backward order); other gates run as
+  // Plan (backward order): runs of consecutive steps become one pass: tile
+  // passes in LDS (default: up to AT_MAX_STEPS steps whose bits fit in a tile
+  // of `tile_bits` amplitude bits with the AT_LOW lowest ones), or register
+  // passes spanning <= 3 amplitude bits (QS_ADJ_TILE=0); other gates run as
   // single steps.
   struct Item {
     int multi;   // 1: register pass index into passes, 2: tile pass index into tpasses, 0: single gate index
@@ -799,50 +795,14 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
     curbits = 0;
   };
   std::vector<int> single_row(ngates, -1);
-  // With the forward records and tile passes, the forward state is computed
-  // while the rest is planned: |0...0>, then the items as they are planned
-  // (circuit order), their steps uploaded in chunks of 1, 2, 4, ... 16 passes.
-  const bool stream_fwd = fwd && tile;
-  size_t ts_up = 0, items_fwd = 0;
-  int fwd_chunk = 1;
-  if (stream_fwd) {
-    rc = qs_sv_init(ctx, psi, n, dt, 0, s);
-    if (rc) return rc;
-  }
-  auto forward_items = [&](bool all) -> int {
-    if (!stream_fwd) return QS_OK;
-    int pending = 0;
-    for (size_t i = items_fwd; i < items.size(); ++i) pending += items[i].multi == 2;
-    if (!all && pending < fwd_chunk) return QS_OK;
-    if (ts.size() > ts_up) {
-      const size_t b = ts_up * sizeof(TStep), e = ts.size() * sizeof(TStep);
-      std::memcpy(hs + ts_off + b, (const char*)ts.data() + b, e - b);
-      QS_CHECK_HIP(hipMemcpyAsync(ds + ts_off + b, hs + ts_off + b, e - b, hipMemcpyHostToDevice, s));
-      ts_up = ts.size();
-    }
-    for (; items_fwd < items.size(); ++items_fwd) {
-      const Item& it = items[items_fwd];
-      if (it.multi == 2) {
-        const TPass& P = tpasses[it.idx];
-        const uint64_t ntiles = 1ull << (n - P.nb);
-        const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
-        hipLaunchKernelGGL(k_fwd_tile<R>, dim3(blocks), dim3(ADJ_THREADS), ((size_t)1 << P.nb) * sizeof(cx<R>), s,
-                           psi, ntiles, P, d_ts);
-        QS_CHECK_HIP(hipGetLastError());
-      } else {
-        const int r = apply_gate(ctx, psi, n, dt, &fwd[it.idx], s);
-        if (r) return r;
-      }
-    }
-    fwd_chunk = std::min(2 * fwd_chunk, 16);
-    return QS_OK;
-  };
   // Tile passes group the steps like the forward fusion planner: within a run
-  // of eligible gates (circuit order), a gate may join the current pass ahead
-  // of gates still waiting if it shares no bit with them (disjoint gates
+  // of eligible steps (backward order), a step may join the current pass ahead
+  // of steps still waiting if it shares no bit with them (disjoint gates
   // commute, so the same U and the gradient of the same function come out),
-  // fewest new tile bits first.  The forward applies a pass's steps in order,
-  // the sweep undoes them last first.
+  // fewest new tile bits first; the pass applies its steps in backward order.
+  // (Planned in circuit order instead, for a forward launched while the rest
+  // is planned, the 20-qubit sweep needed 31 passes of 44.9 us against 32 of
+  // 42.0 us.)
   std::vector<int> run;
   const uint64_t all_bits = n >= 64 ? ~0ull : (1ull << n) - 1;
   auto plan_run = [&]() -> int {
@@ -886,14 +846,11 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
           --left;
         }
       flush_tile();
-      const int r = forward_items(false);
-      if (r) return r;
     }
     run.clear();
     return QS_OK;
   };
-  for (int jj = 0; jj < ngates; ++jj) {
-    const int j = tile ? jj : ngates - 1 - jj;
+  for (int j = ngates - 1; j >= 0; --j) {
     if (tile && eligible(j)) {
       run.push_back(j);
       continue;
@@ -913,7 +870,6 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
         step_slot.push_back(slot[j]);
       }
       items.push_back({0, j});
-      if ((rc = forward_items(false))) return rc;
       continue;
     }
     const uint64_t b = gbits(j);
@@ -923,22 +879,35 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   }
   if (tile && (rc = plan_run())) return rc;
   flush();
-  if ((rc = forward_items(true))) return rc;
   if ((int)nb.size() != nparam || mg.size() > (size_t)(tile ? 0 : ngates) || ts.size() > (size_t)ngates) {
     set_error("adjoint gradient: internal error: plan larger than its buffers");
     return QS_ERR_DEVICE;
   }
-  // the rest of the descriptors: steps not yet uploaded, register-pass gates, block counts
+  // the descriptors: register-pass gates, steps, block counts
   if (!mg.empty()) std::memcpy(hs, mg.data(), mg.size() * sizeof(MGate));
-  if (ts.size() > ts_up) std::memcpy(hs + ts_off + ts_up * sizeof(TStep), ts.data() + ts_up, (ts.size() - ts_up) * sizeof(TStep));
+  if (!ts.empty()) std::memcpy(hs + ts_off, ts.data(), ts.size() * sizeof(TStep));
   if (!nb.empty()) std::memcpy(hs + nb_off, nb.data(), nb.size() * sizeof(int));
   if (!mg.empty()) QS_CHECK_HIP(hipMemcpyAsync(ds, hs, mg.size() * sizeof(MGate), hipMemcpyHostToDevice, s));
-  if (ts.size() > ts_up) {
-    const size_t b = ts_off + ts_up * sizeof(TStep);
-    QS_CHECK_HIP(hipMemcpyAsync(ds + b, hs + b, (ts.size() - ts_up) * sizeof(TStep), hipMemcpyHostToDevice, s));
-  }
+  if (!ts.empty())
+    QS_CHECK_HIP(hipMemcpyAsync(ds + ts_off, hs + ts_off, ts.size() * sizeof(TStep), hipMemcpyHostToDevice, s));
   QS_CHECK_HIP(hipMemcpyAsync(ds + nb_off, hs + nb_off, rows * sizeof(int), hipMemcpyHostToDevice, s));
-  if (tile) std::reverse(items.begin(), items.end());  // the sweep: last pass first
+  if (fwd && tile) {
+    // the forward state: |0...0>, then the plan's items last first
+    rc = qs_sv_init(ctx, psi, n, dt, 0, s);
+    for (auto it = items.rbegin(); !rc && it != items.rend(); ++it) {
+      if (it->multi == 2) {
+        const TPass& P = tpasses[it->idx];
+        const uint64_t ntiles = 1ull << (n - P.nb);
+        const int blocks = (int)std::min<uint64_t>(ADJ_MAX_BLOCKS, ntiles);
+        hipLaunchKernelGGL(k_fwd_tile<R>, dim3(blocks), dim3(ADJ_THREADS), ((size_t)1 << P.nb) * sizeof(cx<R>), s,
+                           psi, ntiles, P, d_ts);
+        QS_CHECK_HIP(hipGetLastError());
+      } else {
+        rc = apply_gate(ctx, psi, n, dt, &fwd[it->idx], s);  // register passes exist only without tiles
+      }
+    }
+    if (rc) return rc;
+  }
   if (energy) {
     rc = pauli_sum_async(ctx, psi, lam, n, dt, terms, nterms, s);
     if (rc) return rc;
