@@ -79,7 +79,35 @@ __global__ __launch_bounds__(256) void k_gate(cx<R>* __restrict__ v, uint64_t ng
   }
 }
 
-// k-target dense mat-vec (k >= 3), out of place: one thread per output amplitude.
+// Three-target dense gate in place: one thread per group of 8 amplitudes,
+// the 8 x 8 matrix and the target offsets in the kernel arguments (no
+// scratch, nothing to keep alive on the host).
+template <typename R> struct Dense3Args {
+  cx<R> m[64];  // row-major; target 0 is the matrix index MSB
+  uint64_t off[8];
+  SortedBits sb;
+  uint64_t cmask;
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_dense3(cx<R>* __restrict__ v, uint64_t groups, const Dense3Args<R> a) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= groups) return;
+  const uint64_t base = insert_zeros(g, a.sb.b, a.sb.n) | a.cmask;
+  cx<R> x[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) x[c] = ld(v + (base | a.off[c]));
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    cx<R> y = mul(a.m[8 * r], x[0]);
+#pragma unroll
+    for (int c = 1; c < 8; ++c) y = cfma(y, a.m[8 * r + c], x[c]);
+    st(v + (base | a.off[r]), y);
+  }
+}
+
+// k-target dense mat-vec (k >= 4 where the matrix-core kernel does not
+// apply), out of place: one thread per output amplitude.
 // out[base(g) | off(r)] = sum_c A[r][c] * in[base(g) | off(c)]
 template <typename R>
 __global__ __launch_bounds__(256) void k_dense_k(const cx<R>* __restrict__ in, cx<R>* __restrict__ out,
@@ -217,12 +245,34 @@ static int launch_dense_k(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* g, hipStr
   return QS_OK;
 }
 
+template <typename R> static int launch_dense3(cx<R>* v, int n, const qs_gate* g, hipStream_t s) {
+  const int c = g->ncontrols;
+  Dense3Args<R> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int i = 0; i < 64; ++i) a.m[i] = mk<R>((R)g->big[2 * i], (R)g->big[2 * i + 1]);
+  for (int m = 0; m < 8; ++m) {
+    uint64_t o = 0;
+    for (int j = 0; j < 3; ++j)
+      if ((m >> (2 - j)) & 1) o |= 1ull << g->bits[c + j];
+    a.off[m] = o;
+  }
+  a.sb.n = c + 3;
+  for (int i = 0; i < c + 3; ++i) a.sb.b[i] = g->bits[i];
+  std::sort(a.sb.b, a.sb.b + a.sb.n);
+  for (int i = 0; i < c; ++i) a.cmask |= 1ull << g->bits[i];
+  const uint64_t groups = 1ull << (n - c - 3);
+  hipLaunchKernelGGL(k_dense3<R>, dim3((unsigned)((groups + 255) / 256)), dim3(256), 0, s, v, groups, a);
+  QS_CHECK_HIP(hipGetLastError());
+  return QS_OK;
+}
+
 template <typename R> int apply_gate_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* g, hipStream_t s) {
   const int t = g->ntargets;
   switch (g->kind) {
     case QS_GATE_DENSE:
       if (t == 1) return launch_kind<R, QS_GATE_DENSE, 1>(v, n, g, s);
       if (t == 2) return launch_kind<R, QS_GATE_DENSE, 2>(v, n, g, s);
+      if (t == 3) return launch_dense3<R>(v, n, g, s);
       if (dense_mfma_eligible(g, n))
         return launch_dense_mfma(ctx, v, n, sizeof(R) == 8 ? QS_C128 : QS_C64, g, s);
       return launch_dense_k<R>(ctx, v, n, g, s);

@@ -84,3 +84,25 @@ def test_matrix_core_dense_at_24_qubits_norm_and_inverse():
     mid = sim.run(c2, state_vector=torch.as_tensor(before, device="cuda:0")).get()
     assert abs(np.linalg.norm(mid) - 1) < 1e-12
     assert np.max(np.abs(mid - before)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("n", [9, 15])
+def test_three_target_dense_in_place_per_gate(n, precision):
+    """Per-gate passes (fusion off, and every gate of a register below the
+    tile size) of three-target dense gates, uncontrolled and with one or two
+    controls, on low, high and scattered bits: the in-place k_dense3 against
+    the oracle."""
+    say("dense3 per gate", n, precision)
+    from quict_amd.simulation import StateVectorSimulator
+    ctl2 = np.block([[np.eye(24), np.zeros((24, 8))], [np.zeros((8, 24)), rand_unitary(3, 8)]])
+    ctl1 = np.block([[np.eye(8), np.zeros((8, 8))], [np.zeros((8, 8)), rand_unitary(3, 9)]])
+    blocks = lambda: [(Unitary(rand_unitary(3, 6)), [n - 1, n - 2, n - 3]), (Unitary(rand_unitary(3, 7)), [0, 1, 2]),
+                      (QFT(3), [4, 1, n - 2]), (Unitary(ctl2), [3, n - 1, 0, 5, 2]),
+                      (Unitary(ctl1), [n - 3, 1, 7, 4])]
+    c = circuit_with(n, blocks(), 61)
+    ref = osv.run(circuit_with(n, blocks(), 61))
+    out = StateVectorSimulator("GPU", precision=precision, fusion=False).run(c).get()
+    err = float(np.max(np.abs(out - ref)))
+    print(f"  max-abs error vs the oracle: {err:.2e}")
+    assert err < (1e-12 if precision == "double" else 2e-5)
