@@ -22,8 +22,9 @@
  *  - stream is a hipStream_t (NULL = the null stream).  Calls that return host
  *    values synchronise that stream.  Gate calls (qs_sv_apply_gate(s),
  *    qs_graph_launch) return once their kernels are queued, except a dense
- *    gate on 3 or more targets run as its own pass (its matrix upload waits
- *    for the stream); the fused program of a qs_sv_apply_gates call stays in
+ *    gate on 4 or more targets run as its own pass (its matrix upload waits
+ *    for the stream; 3-target gates carry theirs in the kernel arguments);
+ *    the fused program of a qs_sv_apply_gates call stays in
  *    one of two per-context slots until its passes finish, so the call after
  *    next on the same context may wait for them.  The adjoint, QNN and
  *    circuit-matrix calls return host values and synchronise.
