@@ -31,7 +31,7 @@ int ensure_scratch(qs_ctx* ctx, uint64_t bytes) {
   return QS_OK;
 }
 
-int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out) {
+int acquire_prog_slot(qs_ctx* ctx, uint64_t host_bytes, uint64_t dev_bytes, qs_ctx::ProgSlot** out) {
   qs_ctx::ProgSlot& S = ctx->prog_slot[ctx->prog_next];
   ctx->prog_next ^= 1;
   if (!S.done) QS_CHECK_HIP(hipEventCreateWithFlags(&S.done, hipEventDisableTiming));
@@ -39,23 +39,28 @@ int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out) {
     QS_CHECK_HIP(hipEventSynchronize(S.done));
     S.pending = false;
   }
-  const uint64_t want = std::max<uint64_t>(bytes, 1ull << 16);
-  if (S.host_bytes < bytes) {
+  if (S.host_bytes < host_bytes) {
     if (S.host) QS_CHECK_HIP(hipHostFree(S.host));
     S.host = nullptr;
     S.host_bytes = 0;
+    const uint64_t want = std::max<uint64_t>(host_bytes, 1ull << 16);
     QS_CHECK_HIP(hipHostMalloc((void**)&S.host, want, hipHostMallocDefault));
     S.host_bytes = want;
   }
-  if (S.dev_bytes < bytes) {
+  if (S.dev_bytes < dev_bytes) {
     if (S.dev) QS_CHECK_HIP(hipFree(S.dev));
     S.dev = nullptr;
     S.dev_bytes = 0;
+    const uint64_t want = std::max<uint64_t>(dev_bytes, 1ull << 16);
     QS_CHECK_HIP(hipMalloc(&S.dev, want));
     S.dev_bytes = want;
   }
   *out = &S;
   return QS_OK;
+}
+
+int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out) {
+  return acquire_prog_slot(ctx, bytes, bytes, out);
 }
 
 int prof_begin(qs_ctx* ctx, int cls, hipStream_t s) {

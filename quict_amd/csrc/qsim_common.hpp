@@ -139,6 +139,8 @@ int ensure_scratch(qs_ctx* ctx, uint64_t bytes);
 // A free program slot of at least `bytes` (host and device), waiting for its
 // previous use to finish.
 int acquire_prog_slot(qs_ctx* ctx, uint64_t bytes, qs_ctx::ProgSlot** out);
+// The same with separate pinned-host and device sizes.
+int acquire_prog_slot(qs_ctx* ctx, uint64_t host_bytes, uint64_t dev_bytes, qs_ctx::ProgSlot** out);
 int check_state_args(const void* vec, int n, int dtype);
 // Profiling: bracket a launch of kernel class `cls` with events (no-op unless enabled).
 int prof_begin(qs_ctx* ctx, int cls, hipStream_t s);

@@ -655,7 +655,7 @@ static int adjoint_t(qs_ctx* ctx, cx<R>* psi, cx<R>* lam, int n, const qs_gate* 
   const uint64_t ip_off = al(sums_off + rows * sizeof(double));
   const uint64_t dev_end = ip_off + (2 * (uint64_t)inner_partial_blocks() + 64) * sizeof(double);
   qs_ctx::ProgSlot* pslot = nullptr;
-  rc = acquire_prog_slot(ctx, std::max<uint64_t>(dev_end, res_off + (rows + 2) * sizeof(double)), &pslot);
+  rc = acquire_prog_slot(ctx, res_off + (rows + 2) * sizeof(double), dev_end, &pslot);
   if (rc) return rc;
   struct SlotGuard {  // the slot is busy until everything enqueued below has run
     qs_ctx::ProgSlot* sl;
