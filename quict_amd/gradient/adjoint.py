@@ -54,8 +54,9 @@ class PauliSum:
         self._recs = None  # (terms they were built from, qs_pauli_term records)
 
     def records(self) -> np.ndarray:
-        if self._recs is not None and self._recs[0] == self.terms:
-            return self._recs[1]
+        cached = getattr(self, "_recs", None)
+        if cached is not None and cached[0] == self.terms:
+            return cached[1]
         arr = (nat.QsPauliTerm * len(self.terms))()
         for k, (c, s) in enumerate(self.terms):
             x = z = 0
@@ -248,7 +249,7 @@ def _records(pc: ParameterizedCircuit, theta: np.ndarray, reuse: bool = False):
     next such call; for callers that hand them to one native call)."""
     fwd0, inv0, der0, slot, keep0, fast, generic, lin = _template(pc)
     if reuse:
-        if pc._work is None or pc._work[0] is not fwd0:
+        if getattr(pc, "_work", None) is None or pc._work[0] is not fwd0:
             pc._work = (fwd0, fwd0.copy(), inv0.copy(), der0.copy())
         fwd, inv, der = pc._work[1:]
     else:
