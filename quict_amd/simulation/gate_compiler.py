@@ -196,30 +196,33 @@ def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: boo
     big = np.zeros(G, dtype=np.uint64)
     fast = row_offset == 0 and not conj and not dagger
     rot = {}   # (type, precision) -> ([slot], [gate])
+    static = {}  # (type, precision) -> [slot]
+    b1k, b1 = [], []              # one-argument gates: slot, argument
+    b2k, b2a, b2b = [], [], []    # two-argument gates: slot, arguments
     k = 0
     for g in gates:
         gt = g.type
-        if fast and gt in _STATIC_TYPES:
+        if fast and (gt in _STATIC_TYPES or gt in _ROTATION_TYPES):
+            args = g.cargs + g.targs
+            if len(args) == 1:
+                b1k.append(k)
+                b1.append(args[0])
+            elif len(args) == 2:
+                b2k.append(k)
+                b2a.append(args[0])
+                b2b.append(args[1])
+            else:
+                bits[k, : len(args)] = [n - 1 - a for a in args]
             key = (gt, g.precision)
-            rec = _STATIC_RECORDS.get(key)
-            if rec is None:
-                r = _translate(g, n, keep, 0, False, False)
-                rec = _STATIC_RECORDS[key] = (r[0], r[1], r[2], None if r[4] is None else
-                                              np.ascontiguousarray(r[4], dtype=np.complex128).view(np.float64))
-            kind[k], nc[k], nt[k] = rec[0], rec[1], rec[2]
-            args = g.cargs + g.targs
-            bits[k, : len(args)] = [n - 1 - a for a in args]
-            if rec[3] is not None:
-                m[k, : len(rec[3])] = rec[3]
-            k += 1
-            continue
-        if fast and gt in _ROTATION_TYPES:
-            args = g.cargs + g.targs
-            nc[k], nt[k] = g.controls, g.targets
-            bits[k, : len(args)] = [n - 1 - a for a in args]
-            lst = rot.setdefault((gt, g.precision), ([], []))
-            lst[0].append(k)
-            lst[1].append(g)
+            if gt in _STATIC_TYPES:
+                lst = static.get(key)
+                if lst is None:
+                    lst = static[key] = [g]  # the first entry: a gate to translate once
+                lst.append(k)
+            else:
+                lst = rot.setdefault(key, ([], []))
+                lst[0].append(k)
+                lst[1].append(g)
             k += 1
             continue
         r = _translate(g, n, keep, row_offset, conj, dagger)
@@ -232,6 +235,23 @@ def compile_gates(gates: List[BasicGate], n: int, row_offset: int = 0, conj: boo
             m[k, : len(v)] = v
         big[k] = r[5]
         k += 1
+    if b1k:
+        bits[b1k, 0] = n - 1 - np.asarray(b1)
+    if b2k:
+        bits[b2k, 0] = n - 1 - np.asarray(b2a)
+        bits[b2k, 1] = n - 1 - np.asarray(b2b)
+    for key, lst in static.items():
+        rec = _STATIC_RECORDS.get(key)
+        if rec is None:
+            r = _translate(lst[0], n, keep, 0, False, False)
+            rec = _STATIC_RECORDS[key] = (r[0], r[1], r[2], None if r[4] is None else
+                                          np.ascontiguousarray(r[4], dtype=np.complex128).view(np.float64))
+        idx = np.asarray(lst[1:])
+        kind[idx], nc[idx], nt[idx] = rec[0], rec[1], rec[2]
+        if rec[3] is not None:
+            m[idx, : len(rec[3])] = rec[3]
+    for (gt, dt), (slots, gl) in rot.items():
+        nc[slots], nt[slots] = gl[0].controls, gl[0].targets
     for (gt, dt), (slots, gl) in rot.items():
         slots = np.asarray(slots)
         P = np.array([g.pargs for g in gl], dtype=np.float64)
