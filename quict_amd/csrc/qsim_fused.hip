@@ -721,11 +721,13 @@ static int launch_passes(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, con
 
 // Planner settings: LR = 0 (low bits stay on threads at the load and store);
 // `moves` lets the planner move qubits between index bits when that saves at
-// least 8% of the passes.  QS_FUSION_REMAP overrides it for every call: 0
-// never, 1 whenever it saves a pass.
-static Config fused_config(int RB, int TB, int L, bool moves) {
+// least 8% of the passes, for lists of more than 480 gates (shorter lists
+// rarely make the 24 passes the moving plan is tried for; they plan without
+// moves and so stream, see run_fused_t).  QS_FUSION_REMAP overrides it for
+// every call: 0 never, 1 whenever it saves a pass.
+static Config fused_config(int RB, int TB, int L, bool moves, int ngates) {
   Config c{RB, TB, L};
-  c.remap = moves ? 1 : 0;
+  c.remap = moves && ngates > 480 ? 1 : 0;
   if (const char* rm = getenv("QS_FUSION_REMAP")) c.remap = rm[0] == '1' ? 2 : 0;
   return c;
 }
@@ -771,8 +773,8 @@ static int run_fused_streamed(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates
 template <typename R, int RB, int TB>
 static int run_fused_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, bool moves,
                        int* passes, hipStream_t s) {
-  const Config c = fused_config(RB, TB, L, moves);
-  if (c.remap == 0) {
+  const Config c = fused_config(RB, TB, L, moves, ngates);
+  if (c.remap == 0) {  // plans without moves are final as they form: streamed
     const char* st = getenv("QS_FUSION_STREAM");
     if (!(st && st[0] == '0')) return run_fused_streamed<R, RB, TB>(ctx, v, n, gates, ngates, c, passes, s);
   }
@@ -821,7 +823,7 @@ template <typename R> __global__ __launch_bounds__(256) void k_basis0(cx<R>* __r
 template <typename R, int RB, int TB>
 static int capture_t(qs_ctx* ctx, cx<R>* v, int n, const qs_gate* gates, int ngates, int L, int init_zero,
                      qs_graph* G) {
-  const Config c = fused_config(RB, TB, L, true);  // state vectors: plans as fuse=2
+  const Config c = fused_config(RB, TB, L, true, ngates);  // state vectors: plans as fuse=2
   FusedProgram prog;
   int rc = build_program(gates, ngates, n, c, (uint32_t)sizeof(cx<R>), prog);
   if (rc) return rc;
