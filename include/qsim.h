@@ -110,7 +110,9 @@ int qs_sv_apply_gate(qs_ctx* ctx, void* vec, int n, int dtype, const qs_gate* ga
    runs each group in one read+write pass.  fuse=2: as 1, and the planner may
    also move qubits onto the low index bits between passes (every qubit is back
    on its own bit when the call returns) when that saves at least 8% of the
-   passes; faster on long state-vector circuits, not on density matrices.
+   passes, for lists of more than 480 gates; faster on long state-vector
+   circuits, not on density matrices.  Plans without moves (fuse=1, and fuse=2
+   lists of up to 480 gates) are launched pass by pass while they are planned.
    Replaces the per-gate loop of StateVectorSimulator.run
    (statevector_simulator.py:155-164, 202-315).
    *passes (may be NULL) receives the number of full-state passes executed. */
